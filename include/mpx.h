@@ -1,0 +1,141 @@
+/*
+ * mpx -- MI355X-native NNEDI3 / RAVU prescalers: the C ABI.
+ *
+ * The reference (bjin/mpv-prescalers) is a set of mpv/libplacebo user shaders; its
+ * "interface" is the `//!` directive block at the head of every pass, which mpv parses
+ * and dispatches.  Each entry point below names the directive lines it replaces.
+ * All pointers are device pointers unless stated otherwise; sizes are in pixels,
+ * pitches and strides in ELEMENTS (not bytes).  No entry point aborts: every failure is
+ * a negative status with a thread-local message from mpx_last_error().
+ */
+#ifndef MPX_H
+#define MPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define MPX_OK 0
+#define MPX_SKIPPED 1             /* WHEN is false for these sizes: output = input copy */
+#define MPX_ERR_ARG (-1)          /* bad argument (null pointer, bad enum, bad stride)  */
+#define MPX_ERR_UNKNOWN_VARIANT (-2)
+#define MPX_ERR_UNAVAILABLE (-3)  /* variant known but its weights are not shipped     */
+#define MPX_ERR_SHAPE (-4)        /* sizes out of range or batch > plan's max_batch    */
+#define MPX_ERR_BUDGET (-5)       /* allocation would exceed the context's HBM budget  */
+#define MPX_ERR_HIP (-6)          /* HIP runtime error (message has the HIP error)     */
+#define MPX_ERR_IO (-7)           /* weight file missing or corrupt                    */
+#define MPX_ERR_UNSUPPORTED (-8)  /* combination not supported (e.g. precision)        */
+
+/* ---- element types of planes ------------------------------------------------------ */
+#define MPX_U8 0    /* 8-bit unorm, value / 255 (HOOKED_mul of an 8-bit texture)          */
+#define MPX_U16 1   /* 16-bit container, value / (2^bits - 1), bits given at plan time   */
+#define MPX_F16 2
+#define MPX_F32 3
+
+/* ---- NNEDI3 contraction precision (RAVU always computes in FP32) ------------------ */
+#define MPX_PREC_FP32 0   /* plain FP32 VALU kernel                                       */
+#define MPX_PREC_FP16 1   /* FP16 MFMA, FP32 accumulation                                 */
+#define MPX_PREC_BF16 2   /* BF16 MFMA, FP32 accumulation                                 */
+#define MPX_PREC_FP8 3    /* FP8 e4m3 MFMA, per-neuron FP32 scales                        */
+
+/* ---- variant families / channel forms --------------------------------------------- */
+#define MPX_FAMILY_NNEDI3 0
+#define MPX_FAMILY_RAVU 1
+#define MPX_FAMILY_RAVU_LITE 2
+#define MPX_FAMILY_RAVU_3X 3
+#define MPX_FAMILY_RAVU_ZOOM 4
+
+#define MPX_CH_LUMA 0   /* //!HOOK LUMA, one plane                                        */
+#define MPX_CH_YUV 1    /* //!HOOK NATIVE, 3 planes, hash from plane 0                    */
+#define MPX_CH_RGB 2    /* //!HOOK MAIN, 3 planes, hash from BT.709 luma                  */
+
+typedef struct mpx_variant_info {
+  int32_t family;       /* MPX_FAMILY_*                                                   */
+  int32_t channels;     /* MPX_CH_*                                                       */
+  int32_t planes;       /* 1 or 3                                                         */
+  int32_t radius;       /* RAVU radius r (0 for NNEDI3)                                   */
+  int32_t nns;          /* NNEDI3 neurons (0 for RAVU)                                    */
+  int32_t win_w, win_h; /* NNEDI3 window (8 x 4 or 8 x 6)                                 */
+  int32_t ar;           /* anti-ringing variant                                           */
+  int32_t available;    /* weights shipped (0 for ravu-zoom-ar-r3*)                       */
+  int32_t applies;      /* WHEN evaluated for the given in/target sizes                   */
+  int32_t out_w, out_h; /* size of the hook's output (in size when it does not apply)     */
+  int32_t scale_x, scale_y; /* integer scale per axis; 0 for arbitrary-ratio (zoom)       */
+  int32_t align;        /* 1 for OFFSET ALIGN (zoom)                                      */
+  float offset_x, offset_y; /* accumulated OFFSET of the output, in output pixels         */
+  char hook[8];         /* "LUMA" | "NATIVE" | "MAIN"                                     */
+} mpx_variant_info;
+
+typedef struct mpx_ctx mpx_ctx;
+typedef struct mpx_plan mpx_plan;
+
+/* Full I/O description for mpx_run_ex.  Strides in elements; planes of one frame are
+ * cstride apart, frames bstride apart. */
+typedef struct mpx_io {
+  const void* in;
+  int64_t in_pitch, in_cstride, in_bstride;
+  void* out;
+  int64_t out_pitch, out_cstride, out_bstride;
+  int32_t batch;
+  int32_t* buckets;     /* optional (may be NULL): per-site LUT row ids, int32, see
+                           mpx_plan_bucket_layout; for parity reporting               */
+} mpx_io;
+
+/* ---- host-only queries (no GPU needed) -------------------------------------------- */
+
+/* Number of shipped variant names and the i-th name (file names without ".hook"). */
+int mpx_variant_count(void);
+const char* mpx_variant_name(int i);
+
+/* Geometry / gating of `name` for an input of in_w x in_h displayed at target_w x
+ * target_h.  Replaces mpv's evaluation of //!WHEN, //!WIDTH, //!HEIGHT, //!OFFSET of the
+ * variant's passes (e.g. ravu-lite-r3.hook:20,123-125; nnedi3-nns256-win8x6.hook:19,
+ * 349-352; ravu-zoom-r2.hook:18-21; compute/ravu-3x-r3.hook:18-20). */
+int mpx_query(const char* name, int in_w, int in_h, int target_w, int target_h,
+              mpx_variant_info* info);
+
+/* ---- context ---------------------------------------------------------------------- */
+
+/* Bind a HIP device, set the HBM budget (bytes; 0 = no limit) for weights + plan
+ * scratch, and the directory holding the shipped *.bin weight files (NULL = the
+ * directory compiled in / MPX_WEIGHTS_DIR).  Replaces mpv's loading of the `.hook`
+ * file and its //!TEXTURE blocks (e.g. ravu-lite-r3.hook:132-136). */
+int mpx_create(int device, uint64_t hbm_budget, const char* weights_dir, mpx_ctx** out);
+void mpx_destroy(mpx_ctx* ctx);
+/* Bytes of device memory the context holds (weights + all live plans' scratch). */
+uint64_t mpx_ctx_bytes(const mpx_ctx* ctx);
+
+/* ---- plans ------------------------------------------------------------------------ */
+
+/* Prepare `name` for fixed sizes and element types: uploads the weights once (kept
+ * resident in the context), allocates scratch for max_batch frames, picks the kernels.
+ * `bits` is the bit depth of MPX_U16 input (ignored otherwise).  Replaces the pass list
+ * of the hook file (//!HOOK, //!BIND, //!SAVE, //!COMPONENTS, //!COMPUTE lines, e.g.
+ * ravu-r3.hook:15-20,148-154,282-288,416-425).  Returns MPX_SKIPPED (and a valid plan
+ * that copies) when //!WHEN is false for these sizes. */
+int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int target_w,
+                    int target_h, int in_type, int bits, int out_type, int precision,
+                    int max_batch, mpx_plan** out);
+int mpx_plan_info(const mpx_plan* plan, mpx_variant_info* info);
+/* Number of int32 bucket ids one frame writes when mpx_io.buckets is set (0 for NNEDI3). */
+int64_t mpx_plan_bucket_count(const mpx_plan* plan);
+void mpx_plan_destroy(mpx_plan* plan);
+
+/* Run the plan on `batch` frames, asynchronously on `hip_stream` (a hipStream_t; NULL =
+ * default stream).  Planes of a frame are contiguous (cstride = pitch * height).
+ * Allocates nothing.  Replaces one frame's dispatch of all the hook's passes (e.g.
+ * ravu-lite-r3.hook:15-131 step1 + step2). */
+int mpx_run(mpx_plan* plan, const void* in, int64_t in_pitch, int64_t in_bstride, void* out,
+            int64_t out_pitch, int64_t out_bstride, int batch, void* hip_stream);
+int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* mpx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPX_H */

@@ -1,0 +1,359 @@
+// The C ABI of include/mpx.h: variant registry, context (device + resident weights +
+// HBM budget), plans (sizes, scratch, kernel choice) and dispatch.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mpx.h"
+#include "launch.h"
+#include "nnedi3.h"
+#include "ravu.h"
+#include "internal.h"
+
+namespace {
+
+// ----------------------------------------------------------------------------- registry
+
+using mpx_host::Variant;
+
+const std::vector<Variant>& registry() {
+  static const std::vector<Variant> v = [] {
+    std::vector<Variant> out;
+    const char* chs[3] = {"", "-yuv", "-rgb"};
+    for (int nns : {16, 32, 64, 128, 256})
+      for (int wh : {4, 6})
+        out.push_back({"nnedi3-nns" + std::to_string(nns) + "-win8x" + std::to_string(wh),
+                       MPX_FAMILY_NNEDI3, MPX_CH_LUMA, 0, nns, wh, 0, 1});
+    for (int r : {2, 3, 4})
+      for (int c = 0; c < 3; ++c)
+        out.push_back({"ravu-r" + std::to_string(r) + chs[c], MPX_FAMILY_RAVU, c, r, 0, 0, 0, 1});
+    for (int ar : {0, 1})
+      for (int r : {2, 3, 4})
+        out.push_back({std::string(ar ? "ravu-lite-ar-r" : "ravu-lite-r") + std::to_string(r),
+                       MPX_FAMILY_RAVU_LITE, MPX_CH_LUMA, r, 0, 0, ar, 1});
+    for (int r : {2, 3, 4})
+      for (int c = 0; c < 3; ++c)
+        out.push_back({"ravu-3x-r" + std::to_string(r) + chs[c], MPX_FAMILY_RAVU_3X, c, r, 0, 0, 0, 1});
+    for (int ar : {0, 1})
+      for (int r : {2, 3})
+        for (int c = 0; c < 3; ++c)
+          // ravu-zoom-ar-r3*: listed in .MISSING_LARGE_BLOBS:1-7, weights not shipped
+          out.push_back({std::string(ar ? "ravu-zoom-ar-r" : "ravu-zoom-r") + std::to_string(r) + chs[c],
+                         MPX_FAMILY_RAVU_ZOOM, c, r, 0, 0, ar, (ar && r == 3) ? 0 : 1});
+    return out;
+  }();
+  return v;
+}
+
+const Variant* find_variant(const char* name) {
+  if (!name) return nullptr;
+  for (const Variant& v : registry())
+    if (v.name == name) return &v;
+  return nullptr;
+}
+
+// mpv evaluates //!WHEN in float: `HOOKED.w OUTPUT.w / 0.707106 <`.
+bool ratio_below(int in, int target, float thr) {
+  return static_cast<float>(in) / static_cast<float>(target) < thr;
+}
+
+void fill_info(const Variant& v, int in_w, int in_h, int tw, int th, mpx_variant_info* info) {
+  std::memset(info, 0, sizeof(*info));
+  info->family = v.family;
+  info->channels = v.channels;
+  info->planes = v.channels == MPX_CH_LUMA ? 1 : 3;
+  info->radius = v.radius;
+  info->nns = v.nns;
+  info->win_w = v.nns ? 8 : 0;
+  info->win_h = v.win_h;
+  info->ar = v.ar;
+  info->available = v.available;
+  std::strcpy(info->hook, v.channels == MPX_CH_LUMA ? "LUMA" : (v.channels == MPX_CH_YUV ? "NATIVE" : "MAIN"));
+  info->out_w = in_w;
+  info->out_h = in_h;
+  switch (v.family) {
+    case MPX_FAMILY_NNEDI3: {
+      // double_y: nnedi3-nns256-win8x6.hook:19; double_x: :364
+      const bool ay = ratio_below(in_h, th, 0.707106f), ax = ratio_below(in_w, tw, 0.707106f);
+      info->applies = ax || ay;
+      info->scale_x = ax ? 2 : 1;
+      info->scale_y = ay ? 2 : 1;
+      info->out_w = in_w * info->scale_x;
+      info->out_h = in_h * info->scale_y;
+      info->offset_x = ax ? -0.5f : 0.f;  // combine_x OFFSET -0.5 0 (:697)
+      info->offset_y = ay ? -0.5f : 0.f;  // combine_y OFFSET 0 -0.5 (:351)
+      break;
+    }
+    case MPX_FAMILY_RAVU:
+    case MPX_FAMILY_RAVU_LITE: {
+      // ravu-r3.hook:20, ravu-lite-r3.hook:20 (both axes, product of the two tests)
+      info->applies = ratio_below(in_w, tw, 0.707106f) && ratio_below(in_h, th, 0.707106f);
+      info->scale_x = info->scale_y = info->applies ? 2 : 1;
+      info->out_w = in_w * info->scale_x;
+      info->out_h = in_h * info->scale_y;
+      if (v.family == MPX_FAMILY_RAVU && info->applies) info->offset_x = info->offset_y = -0.5f;  // ravu-r3.hook:424
+      break;
+    }
+    case MPX_FAMILY_RAVU_3X:
+      // compute/ravu-3x-r3.hook:21
+      info->applies = ratio_below(in_w, tw, 0.471404f) && ratio_below(in_h, th, 0.471404f);
+      info->scale_x = info->scale_y = info->applies ? 3 : 1;
+      info->out_w = in_w * info->scale_x;
+      info->out_h = in_h * info->scale_y;
+      break;
+    case MPX_FAMILY_RAVU_ZOOM:
+      // ravu-zoom-r2.hook:18-21: WIDTH OUTPUT.w, HEIGHT OUTPUT.h, OFFSET ALIGN, in < out
+      info->applies = in_w < tw && in_h < th;
+      info->scale_x = info->scale_y = info->applies ? 0 : 1;
+      if (info->applies) {
+        info->out_w = tw;
+        info->out_h = th;
+        info->align = 1;
+      }
+      break;
+  }
+}
+
+std::string default_weights_dir() {
+  if (const char* e = std::getenv("MPX_WEIGHTS_DIR")) return e;
+  Dl_info dl;
+  if (dladdr(reinterpret_cast<void*>(&default_weights_dir), &dl) && dl.dli_fname) {
+    std::string p(dl.dli_fname);
+    auto pos = p.rfind('/');
+    std::string dir = pos == std::string::npos ? "." : p.substr(0, pos);
+    return dir + "/../weights";
+  }
+  return "weights";
+}
+
+int elem_size(int t) { return t == MPX_F32 ? 4 : (t == MPX_U8 ? 1 : 2); }
+
+}  // namespace
+
+// ----------------------------------------------------------------------------- context
+
+extern "C" {
+
+int mpx_variant_count(void) { return static_cast<int>(registry().size()); }
+
+const char* mpx_variant_name(int i) {
+  if (i < 0 || i >= mpx_variant_count()) return nullptr;
+  return registry()[i].name.c_str();
+}
+
+int mpx_query(const char* name, int in_w, int in_h, int target_w, int target_h, mpx_variant_info* info) {
+  const Variant* v = find_variant(name);
+  if (!v) return fail(MPX_ERR_UNKNOWN_VARIANT, std::string("unknown variant '") + (name ? name : "(null)") + "'");
+  if (!info) return fail(MPX_ERR_ARG, "info is NULL");
+  if (in_w <= 0 || in_h <= 0 || target_w <= 0 || target_h <= 0) return fail(MPX_ERR_SHAPE, "sizes must be positive");
+  fill_info(*v, in_w, in_h, target_w, target_h, info);
+  return MPX_OK;
+}
+
+int mpx_create(int device, uint64_t hbm_budget, const char* weights_dir, mpx_ctx** out) {
+  if (!out) return fail(MPX_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(MPX_ERR_ARG, "device " + std::to_string(device) + " out of range");
+  HIP_TRY(hipSetDevice(device));
+  auto* c = new mpx_ctx();
+  c->device = device;
+  c->budget = hbm_budget;
+  c->wdir = weights_dir ? weights_dir : default_weights_dir();
+  *out = c;
+  return MPX_OK;
+}
+
+void mpx_destroy(mpx_ctx* ctx) {
+  if (!ctx) return;
+  for (auto& kv : ctx->weights) ctx->release(kv.second.ptr, kv.second.bytes);
+  delete ctx;
+}
+
+uint64_t mpx_ctx_bytes(const mpx_ctx* ctx) { return ctx ? ctx->used : 0; }
+
+int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int target_w, int target_h, int in_type,
+                    int bits, int out_type, int precision, int max_batch, mpx_plan** out) {
+  if (!ctx || !out) return fail(MPX_ERR_ARG, "ctx/out is NULL");
+  *out = nullptr;
+  const Variant* v = find_variant(name);
+  if (!v) return fail(MPX_ERR_UNKNOWN_VARIANT, std::string("unknown variant '") + (name ? name : "(null)") + "'");
+  if (!v->available)
+    return fail(MPX_ERR_UNAVAILABLE, std::string(name) +
+                                         ": weights are not shipped by the reference (listed in .MISSING_LARGE_BLOBS)");
+  if (in_w <= 0 || in_h <= 0 || target_w <= 0 || target_h <= 0 || in_w > 32768 || in_h > 32768 ||
+      target_w > 65536 || target_h > 65536)
+    return fail(MPX_ERR_SHAPE, "sizes out of range");
+  if (in_type < MPX_U8 || in_type > MPX_F32) return fail(MPX_ERR_ARG, "bad in_type");
+  if (out_type != MPX_F16 && out_type != MPX_F32) return fail(MPX_ERR_ARG, "out_type must be MPX_F16 or MPX_F32");
+  if (max_batch < 1 || max_batch > 65535) return fail(MPX_ERR_SHAPE, "max_batch out of range");
+  if (in_type == MPX_U16 && (bits < 1 || bits > 16)) return fail(MPX_ERR_ARG, "bits must be 1..16 for MPX_U16");
+  if (precision < MPX_PREC_FP32 || precision > MPX_PREC_FP8) return fail(MPX_ERR_ARG, "bad precision");
+  if (v->family != MPX_FAMILY_NNEDI3 && precision != MPX_PREC_FP32)
+    return fail(MPX_ERR_UNSUPPORTED, "RAVU variants compute in FP32 only");
+
+  std::unique_ptr<mpx_plan> p(new mpx_plan());
+  p->ctx = ctx;
+  p->v = v;
+  fill_info(*v, in_w, in_h, target_w, target_h, &p->info);
+  p->in_w = in_w;
+  p->in_h = in_h;
+  p->in_type = in_type;
+  p->out_type = out_type;
+  p->precision = precision;
+  p->max_batch = max_batch;
+  p->in_max = in_type == MPX_U8 ? 255.f : (in_type == MPX_U16 ? static_cast<float>((1u << bits) - 1u) : 1.f);
+  const bool applies = p->info.applies != 0;
+  int rc = MPX_OK;
+  if (applies) {
+    const std::string r = std::to_string(v->radius);
+    switch (v->family) {
+      case MPX_FAMILY_RAVU_LITE:
+        rc = ctx->lut16("ravu_lite_lut" + r, &p->lut, nullptr);
+        break;
+      case MPX_FAMILY_RAVU:
+        rc = ctx->lut16("ravu_lut" + r, &p->lut, nullptr);
+        p->scratch_frame = static_cast<int64_t>(in_w) * in_h * p->info.planes;  // ravu_int11, FP32
+        break;
+      case MPX_FAMILY_RAVU_3X:
+        rc = ctx->lut16("ravu_3x_lut" + r, &p->lut, nullptr);
+        break;
+      case MPX_FAMILY_RAVU_ZOOM:
+        rc = ctx->lut16("ravu_zoom_lut" + r, &p->lut, nullptr);
+        if (!rc && v->ar) rc = ctx->lut16("ravu_zoom_lut" + r + "_ar", &p->lut_ar, nullptr);
+        if (!rc) rc = mpx::zoom_plan_create(ctx, p.get(), in_w, in_h, p->info.out_w, p->info.out_h);
+        break;
+      case MPX_FAMILY_NNEDI3:
+        rc = mpx::nnedi3_prepare(ctx, v->nns, v->win_h, precision, &p->nw);
+        if (p->info.scale_x == 2 && p->info.scale_y == 2)
+          p->scratch_frame = static_cast<int64_t>(in_w) * in_h * 2;  // double_y output, FP32
+        break;
+    }
+  }
+  if (rc) return rc;
+  if (p->scratch_frame) {
+    p->scratch_bytes = static_cast<size_t>(p->scratch_frame) * 4 * max_batch;
+    rc = ctx->alloc(p->scratch_bytes, &p->scratch);
+    if (rc) return rc;
+  }
+  *out = p.release();
+  return applies ? MPX_OK : MPX_SKIPPED;
+}
+
+int mpx_plan_info(const mpx_plan* plan, mpx_variant_info* info) {
+  if (!plan || !info) return fail(MPX_ERR_ARG, "plan/info is NULL");
+  *info = plan->info;
+  return MPX_OK;
+}
+
+int64_t mpx_plan_bucket_count(const mpx_plan* plan) {
+  if (!plan || !plan->info.applies) return 0;
+  const int64_t px = static_cast<int64_t>(plan->in_w) * plan->in_h;
+  switch (plan->v->family) {
+    case MPX_FAMILY_RAVU_LITE:
+    case MPX_FAMILY_RAVU_3X: return px;
+    case MPX_FAMILY_RAVU: return 3 * px;  // int11, int10, int01 sites
+    case MPX_FAMILY_RAVU_ZOOM: return static_cast<int64_t>(plan->info.out_w) * plan->info.out_h;
+    default: return 0;
+  }
+}
+
+void mpx_plan_destroy(mpx_plan* plan) {
+  if (!plan) return;
+  if (plan->scratch) plan->ctx->release(plan->scratch, plan->scratch_bytes);
+  mpx::zoom_plan_destroy(plan->ctx, plan);
+  delete plan;
+}
+
+int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
+  if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
+  if (!io->in || !io->out) return fail(MPX_ERR_ARG, "in/out is NULL");
+  if (io->batch < 1 || io->batch > plan->max_batch)
+    return fail(MPX_ERR_SHAPE, "batch " + std::to_string(io->batch) + " outside 1.." + std::to_string(plan->max_batch));
+  const mpx_variant_info& inf = plan->info;
+  if (io->in_pitch < plan->in_w || io->out_pitch < inf.out_w)
+    return fail(MPX_ERR_ARG, "pitch smaller than the row width");
+  if (inf.planes > 1 && (io->in_cstride < io->in_pitch * plan->in_h || io->out_cstride < io->out_pitch * inf.out_h))
+    return fail(MPX_ERR_ARG, "plane stride smaller than a plane");
+  if (io->batch > 1 && (io->in_bstride < io->in_pitch * plan->in_h * inf.planes ||
+                        io->out_bstride < io->out_pitch * inf.out_h * inf.planes))
+    return fail(MPX_ERR_ARG, "frame stride smaller than a frame");
+  if (io->buckets && !inf.applies) return fail(MPX_ERR_ARG, "buckets requested from a skipped plan");
+  HIP_TRY(hipSetDevice(plan->ctx->device));
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  mpx::Io d{};
+  d.in = io->in;
+  d.out = io->out;
+  d.in_pitch = io->in_pitch;
+  d.in_cstride = io->in_cstride;
+  d.in_bstride = io->in_bstride;
+  d.out_pitch = io->out_pitch;
+  d.out_cstride = io->out_cstride;
+  d.out_bstride = io->out_bstride;
+  d.buckets = io->buckets;
+  d.bucket_bstride = mpx_plan_bucket_count(plan);
+  d.in_dt = plan->in_type;
+  d.out_dt = plan->out_type;
+  d.in_max = plan->in_max;
+  d.W = plan->in_w;
+  d.H = plan->in_h;
+  d.batch = io->batch;
+  hipError_t e = hipSuccess;
+  if (!inf.applies) {
+    e = mpx::launch_copy(d, inf.planes, st);
+  } else {
+    const Variant& v = *plan->v;
+    switch (v.family) {
+      case MPX_FAMILY_RAVU_LITE:
+        e = mpx::launch_ravu_lite(v.radius, v.ar != 0, d, plan->lut, st);
+        break;
+      case MPX_FAMILY_RAVU_3X:
+        e = mpx::launch_ravu_3x(v.radius, v.channels, d, plan->lut, st);
+        break;
+      case MPX_FAMILY_RAVU:
+        e = mpx::launch_ravu(v.radius, v.channels, d, plan->lut, static_cast<float*>(plan->scratch),
+                             plan->scratch_frame, st);
+        break;
+      case MPX_FAMILY_RAVU_ZOOM:
+        e = mpx::launch_ravu_zoom(v.radius, v.ar != 0, v.channels, d, plan->lut, plan->lut_ar, plan->zoom,
+                                  inf.out_w, inf.out_h, st);
+        break;
+      case MPX_FAMILY_NNEDI3:
+        e = mpx::launch_nnedi3(plan->nw, d, inf.scale_x == 2, inf.scale_y == 2, static_cast<float*>(plan->scratch),
+                               plan->scratch_frame, st);
+        break;
+    }
+  }
+  if (e != hipSuccess) return fail(MPX_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return MPX_OK;
+}
+
+int mpx_run(mpx_plan* plan, const void* in, int64_t in_pitch, int64_t in_bstride, void* out, int64_t out_pitch,
+            int64_t out_bstride, int batch, void* hip_stream) {
+  if (!plan) return fail(MPX_ERR_ARG, "plan is NULL");
+  mpx_io io{};
+  io.in = in;
+  io.in_pitch = in_pitch;
+  io.in_cstride = in_pitch * plan->in_h;
+  io.in_bstride = in_bstride;
+  io.out = out;
+  io.out_pitch = out_pitch;
+  io.out_cstride = out_pitch * plan->info.out_h;
+  io.out_bstride = out_bstride;
+  io.batch = batch;
+  io.buckets = nullptr;
+  return mpx_run_ex(plan, &io, hip_stream);
+}
+
+const char* mpx_last_error(void) { return mpx_host::last_error().c_str(); }
+
+}  // extern "C"

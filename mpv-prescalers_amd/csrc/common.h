@@ -1,0 +1,239 @@
+// Shared device code for the RAVU family: element I/O, the structure tensor, the
+// eigen-analysis hash with its FP64 re-decision near bucket boundaries.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gauss_tables.h"
+
+namespace mpx {
+
+enum DType : int { DT_U8 = 0, DT_U16 = 1, DT_F16 = 2, DT_F32 = 3 };
+
+// Device-side description of one launch's planes (strides in elements).
+struct Io {
+  const void* in;
+  void* out;
+  int64_t in_pitch, in_cstride, in_bstride;
+  int64_t out_pitch, out_cstride, out_bstride;
+  int32_t* buckets;   // optional, [batch][sites] rows
+  int64_t bucket_bstride;
+  int in_dt, out_dt;
+  float in_max;       // integer full-scale value for U8/U16 inputs (255, 1023, ...)
+  int W, H;           // input plane size
+  int batch;
+};
+
+__device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, float in_max) {
+  switch (dt) {
+    case DT_F32: return static_cast<const float*>(p)[idx];
+    case DT_F16: return __half2float(static_cast<const __half*>(p)[idx]);
+    case DT_U8: return static_cast<float>(static_cast<const uint8_t*>(p)[idx]) / in_max;
+    default: return static_cast<float>(static_cast<const uint16_t*>(p)[idx]) / in_max;
+  }
+}
+
+__device__ __forceinline__ void store_elem(void* p, int64_t idx, int dt, float v) {
+  if (dt == DT_F32)
+    static_cast<float*>(p)[idx] = v;
+  else
+    static_cast<__half*>(p)[idx] = __float2half_rn(v);
+}
+
+// Two horizontally adjacent outputs (idx even-aligned by construction of the callers).
+__device__ __forceinline__ void store_pair(void* p, int64_t idx, int dt, float v0, float v1) {
+  if (dt == DT_F32) {
+    float* q = static_cast<float*>(p) + idx;
+    q[0] = v0;
+    q[1] = v1;
+  } else {
+    __half* q = static_cast<__half*>(p) + idx;
+    q[0] = __float2half_rn(v0);
+    q[1] = __float2half_rn(v1);
+  }
+}
+
+__device__ __forceinline__ float4 load_lut4(const __half* lut, int64_t texel) {
+  const __half2* p = reinterpret_cast<const __half2*>(lut + 4 * texel);
+  float2 lo = __half22float2(p[0]);
+  float2 hi = __half22float2(p[1]);
+  return make_float4(lo.x, lo.y, hi.x, hi.y);
+}
+
+template <int N>
+__device__ __host__ constexpr double gauss_w(int i) {
+  return N == 3 ? kGauss3[i] : N == 4 ? kGauss4[i] : N == 5 ? kGauss5[i]
+       : N == 6 ? kGauss6[i] : N == 7 ? kGauss7[i] : kGauss8[i];
+}
+
+// Derivative along one window line at position p, in the shaders' formula forms
+// (ravu-lite-r3.hook:50 central; ravu-r3.hook:63 five-point; ravu-zoom-r2.hook:50
+// forward/backward at the window border).  `s(k)` returns the line's k-th sample.
+template <int N, bool FIVE, typename T, typename Line>
+__device__ __forceinline__ T deriv(Line s, int p) {
+#pragma clang fp contract(off)
+  if (p == 0) return s(1) - s(0);
+  if (p == N - 1) return s(N - 1) - s(N - 2);
+  if (FIVE && p >= 2 && p + 2 <= N - 1)
+    return (-s(p + 2) + T(8) * s(p + 1) - T(8) * s(p - 1) + s(p - 2)) / T(12);
+  return (s(p + 1) - s(p - 1)) / T(2);
+}
+
+// abd = sum over the gradient grid of (gx*gx, gx*gy, gy*gy) * G, grid X outer, Y inner
+// (ravu-lite-r3.hook:48-76).  get(i) returns window sample i = X*N + Y.
+template <int N, bool FIVE, typename T, typename Get>
+__device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
+#pragma clang fp contract(off)
+  constexpr int P0 = N <= 4 ? 0 : 1;
+  constexpr int P1 = N <= 4 ? N : N - 1;
+  a = b = d = T(0);
+#pragma unroll
+  for (int X = P0; X < P1; ++X) {
+#pragma unroll
+    for (int Y = P0; Y < P1; ++Y) {
+      const int gi = (X - P0) * (P1 - P0) + (Y - P0);
+      T gx = deriv<N, FIVE, T>([&](int k) { return get(k * N + Y); }, X);
+      T gy = deriv<N, FIVE, T>([&](int k) { return get(X * N + k); }, Y);
+      const T w = T(gauss_w<N>(gi));
+      a = a + gx * gx * w;
+      b = b + gx * gy * w;
+      d = d + gy * gy * w;
+    }
+  }
+}
+
+enum Rule : int { RULE_LITE = 0, RULE_3X = 1, RULE_LOG2 = 2 };
+
+template <int RULE> struct RuleInfo;
+template <> struct RuleInfo<RULE_LITE> { static constexpr int S = 4; };   // ravu-lite-r3.hook:86,88
+template <> struct RuleInfo<RULE_3X> { static constexpr int S = 3; };     // compute/ravu-3x-r3.hook:95,97
+template <> struct RuleInfo<RULE_LOG2> { static constexpr int S = 9; };   // ravu-r3.hook:117,119
+
+struct Bucket {
+  int angle, strength, coherence;
+};
+
+constexpr float kEps = 1.192092896e-7f;
+constexpr double kEpsD = 1.192092896e-7;
+constexpr double kPiD = 3.141592653589793;
+constexpr float kPiF = 3.141592653589793f;
+
+template <int RULE, typename T>
+__device__ __forceinline__ int strength_of(T lambda) {
+  if (RULE == RULE_LITE)  // ravu-lite-r3.hook:86
+    return lambda >= T(0.016) ? (lambda >= T(0.05) ? 3 : 2) : (lambda >= T(0.004) ? 1 : 0);
+  if (RULE == RULE_3X)    // compute/ravu-3x-r3.hook:95
+    return lambda >= T(0.02) ? 2 : (lambda >= T(0.005) ? 1 : 0);
+  // ravu-r3.hook:117: clamp(floor(log2(lambda*2000 + eps)), 0, 8)
+  T f = floor(log2(lambda * T(2000) + T(1.192092896e-7)));
+  return static_cast<int>(fmin(fmax(f, T(0)), T(8)));
+}
+
+// Eigen-analysis and bucket exactly as the reference writes it (ravu-lite-r3.hook:77-87),
+// in the reference's operation order, no contraction.  Used in FP64 for re-decisions;
+// mirrors the FP64 oracle operation for operation.
+template <int RULE, typename T>
+__device__ __forceinline__ Bucket hash_reference(T a, T b, T d) {
+#pragma clang fp contract(off)
+  const T eps = T(1.192092896e-7);
+  const T pi = T(3.141592653589793);
+  T Tr = a + d;
+  T D = a * d - b * b;
+  T delta = sqrt(fmax(Tr * Tr / T(4) - D, T(0)));
+  T L1 = Tr / T(2) + delta;
+  T L2 = Tr / T(2) - delta;
+  T sqrtL1 = sqrt(L1);
+  T sqrtL2 = sqrt(L2);  // NaN when L2 < 0, as in the reference
+  T at = atan2(L1 - a, b) + pi;
+  T theta = at - pi * floor(at / pi);  // GLSL mod
+  if (fabs(b) < eps) theta = T(0);
+  T ssum = sqrtL1 + sqrtL2;
+  T mu = (sqrtL1 - sqrtL2) / ssum;
+  if (ssum < eps) mu = T(0);
+  Bucket bk;
+  bk.angle = static_cast<int>(floor(theta * T(24) / pi));
+  bk.strength = strength_of<RULE, T>(sqrtL1);
+  bk.coherence = mu >= T(0.5) ? 2 : (mu >= T(0.25) ? 1 : 0);  // NaN -> 0
+  return bk;
+}
+
+// FP32 fast path.  Same quantities through cancellation-free forms (mathematically equal
+// to the reference's): delta = sqrt(((a-d)/2)^2 + b^2) instead of sqrt(T^2/4 - D), and
+// L1 - a = b^2 / ((a-d)/2 + delta) when d < a.  Sets *near when a decision lies within a
+// relative 1e-5 of its bucket boundary, widened for the angle by an FP32 error bound of
+// theta (~16 ulp(T) / (L1 - L2)); exact cases (flat window T = 0, b = 0 exactly) are
+// never near.
+template <int RULE>
+__device__ __forceinline__ Bucket hash_fast(float a, float b, float d, bool* near) {
+  const float eps = 1.192092896e-7f;
+  const float pi = 3.141592653589793f;
+  const float T = a + d;
+  const float h = (a - d) * 0.5f;
+  const float delta = sqrtf(h * h + b * b);
+  const float L1 = T * 0.5f + delta;
+  const float L2 = T * 0.5f - delta;
+  const float y = h <= 0.f ? delta - h : b * b / (h + delta);  // L1 - a >= 0
+  const float sqrtL1 = sqrtf(L1);
+  const float sqrtL2 = sqrtf(L2);
+  const float at = atan2f(y, b) + pi;
+  float theta = at - pi * floorf(at / pi);
+  const bool bsmall = fabsf(b) < eps;
+  if (bsmall) theta = 0.f;
+  const float ssum = sqrtL1 + sqrtL2;
+  float mu = (sqrtL1 - sqrtL2) / ssum;
+  if (ssum < eps) mu = 0.f;
+  const float u = theta * 24.0f / pi;
+  Bucket bk;
+  bk.angle = static_cast<int>(floorf(u));
+  bk.strength = strength_of<RULE, float>(sqrtL1);
+  bk.coherence = mu >= 0.5f ? 2 : (mu >= 0.25f ? 1 : 0);
+  const float rel = 1e-5f, ulp16 = 16.0f * 5.9604645e-8f;
+  bool n = fabsf(fabsf(b) - eps) <= rel * eps + ulp16 * T;
+  n |= fabsf(L2) <= (rel + ulp16) * T && !(b == 0.f && fminf(a, d) == 0.f);
+  if (!bsmall) {
+    const float cond = (24.0f / pi) * ulp16 * T / fmaxf(2.0f * delta, 1e-30f);
+    n |= fabsf(u - rintf(u)) <= 24.0f * rel + cond;
+  }
+  if (RULE == RULE_LITE) {
+    n |= fabsf(sqrtL1 - 0.004f) <= rel * 0.004f || fabsf(sqrtL1 - 0.016f) <= rel * 0.016f ||
+         fabsf(sqrtL1 - 0.05f) <= rel * 0.05f;
+  } else if (RULE == RULE_3X) {
+    n |= fabsf(sqrtL1 - 0.005f) <= rel * 0.005f || fabsf(sqrtL1 - 0.02f) <= rel * 0.02f;
+  } else {
+    const float f = log2f(sqrtL1 * 2000.0f + eps);
+    n |= f > 0.5f && f < 8.5f && fabsf(f - rintf(f)) <= rel * 8.0f;
+  }
+  n |= fabsf(mu - 0.25f) <= rel || fabsf(mu - 0.5f) <= rel;
+  *near = n && T > 0.f;
+  return bk;
+}
+
+// LUT row of a bucket as NEAREST + clamp-to-edge sampling hits it:
+// texel row floor(((angle*S + strength)*3 + coherence + 0.5)) clamped to [0, rows-1]
+// (angle can be 24 or -1 at the wrap of mod(atan(..)+pi, pi)).
+template <int RULE>
+__device__ __forceinline__ int bucket_row(Bucket bk, int rows) {
+  int row = (bk.angle * RuleInfo<RULE>::S + bk.strength) * 3 + bk.coherence;
+  return min(max(row, 0), rows - 1);
+}
+
+// Hash of one site: FP32, re-decided in FP64 when near a boundary.  getf(i) -> float
+// and getd(i) -> double return window sample i (getd widens the same stored values, and
+// for RGB forms it recomputes the BT.709 luma in FP64).
+template <int N, bool FIVE, int RULE, typename GetF, typename GetD>
+__device__ __forceinline__ Bucket ravu_hash(GetF getf, GetD getd) {
+  float a, b, d;
+  structure_tensor<N, FIVE, float>(getf, a, b, d);
+  bool near = false;
+  Bucket bk = hash_fast<RULE>(a, b, d, &near);
+  if (near) {
+    double ad, bd, dd;
+    structure_tensor<N, FIVE, double>(getd, ad, bd, dd);
+    bk = hash_reference<RULE, double>(ad, bd, dd);
+  }
+  return bk;
+}
+
+}  // namespace mpx

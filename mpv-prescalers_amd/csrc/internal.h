@@ -1,0 +1,162 @@
+// Host-side internals shared by the C ABI and the per-family host code: error state,
+// the context (device, HBM budget, resident weights) and the plan.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mpx.h"
+#include "nnedi3.h"
+#include "ravu.h"
+
+namespace mpx_host {
+
+inline std::string& last_error() {
+  thread_local std::string e;
+  return e;
+}
+
+inline int fail(int code, const std::string& msg) {
+  last_error() = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return ::mpx_host::fail(MPX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline bool read_weight_file(const std::string& path, std::vector<float>& data, std::vector<uint32_t>& dims) {
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  char magic[4];
+  uint32_t hdr[3];
+  bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "MPXW", 4) == 0 &&
+            std::fread(hdr, 4, 3, f) == 3 && hdr[0] == 1 && hdr[1] == 0 && hdr[2] >= 1 && hdr[2] <= 4;
+  if (ok) {
+    dims.resize(hdr[2]);
+    ok = std::fread(dims.data(), 4, hdr[2], f) == hdr[2];
+  }
+  if (ok) {
+    size_t n = 1;
+    for (uint32_t d : dims) n *= d;
+    data.resize(n);
+    ok = std::fread(data.data(), 4, n, f) == n;
+  }
+  std::fclose(f);
+  return ok;
+}
+
+struct Variant {
+  std::string name;
+  int family, channels, radius, nns, win_h, ar, available;
+};
+
+}  // namespace mpx_host
+
+using mpx_host::fail;
+
+struct DeviceBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct mpx_ctx {
+  int device = 0;
+  uint64_t budget = 0;
+  uint64_t used = 0;
+  std::string wdir;
+  std::mutex mu;
+  std::map<std::string, DeviceBuf> weights;            // resident weight tables
+  std::map<std::string, std::vector<uint32_t>> dims;   // their host-side shapes
+
+  int alloc(size_t bytes, void** out) {
+    if (budget && used + bytes > budget)
+      return fail(MPX_ERR_BUDGET, "allocation of " + std::to_string(bytes) + " B exceeds the HBM budget (" +
+                                      std::to_string(used) + " of " + std::to_string(budget) + " B used)");
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipMalloc(out, bytes));
+    used += bytes;
+    return MPX_OK;
+  }
+  void release(void* p, size_t bytes) {
+    if (!p) return;
+    (void)hipSetDevice(device);
+    (void)hipFree(p);
+    used -= bytes;
+  }
+
+  // FP16 copy (RNE, as the rgba16f upload) of a LUT weight file, resident on the device.
+  int lut16(const std::string& name, const __half** out, std::vector<uint32_t>* shape) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = weights.find(name);
+    if (it == weights.end()) {
+      std::vector<float> data;
+      std::vector<uint32_t> d;
+      if (!mpx_host::read_weight_file(wdir + "/" + name + ".bin", data, d))
+        return fail(MPX_ERR_IO, "cannot read weight file " + wdir + "/" + name + ".bin");
+      std::vector<_Float16> h(data.size());
+      for (size_t i = 0; i < data.size(); ++i) h[i] = static_cast<_Float16>(data[i]);
+      void* p = nullptr;
+      int rc = alloc(h.size() * 2, &p);
+      if (rc) return rc;
+      HIP_TRY(hipMemcpy(p, h.data(), h.size() * 2, hipMemcpyHostToDevice));
+      it = weights.emplace(name, DeviceBuf{p, h.size() * 2}).first;
+      dims[name] = d;
+    }
+    *out = static_cast<const __half*>(it->second.ptr);
+    if (shape) *shape = dims[name];
+    return MPX_OK;
+  }
+
+  // Host copy of a float32 weight file (NNEDI3 tables are repacked per precision).
+  int host_table(const std::string& name, std::vector<float>& data, std::vector<uint32_t>& d) {
+    if (!mpx_host::read_weight_file(wdir + "/" + name + ".bin", data, d))
+      return fail(MPX_ERR_IO, "cannot read weight file " + wdir + "/" + name + ".bin");
+    return MPX_OK;
+  }
+
+  // Upload an already-packed device table once under `key`.
+  int upload(const std::string& key, const void* src, size_t bytes, const void** out) {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = weights.find(key);
+    if (it == weights.end()) {
+      void* p = nullptr;
+      int rc = alloc(bytes, &p);
+      if (rc) return rc;
+      HIP_TRY(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+      it = weights.emplace(key, DeviceBuf{p, bytes}).first;
+    }
+    *out = it->second.ptr;
+    return MPX_OK;
+  }
+  bool has(const std::string& key) {
+    std::lock_guard<std::mutex> lk(mu);
+    return weights.count(key) != 0;
+  }
+};
+
+struct mpx_plan {
+  mpx_ctx* ctx = nullptr;
+  const mpx_host::Variant* v = nullptr;
+  mpx_variant_info info{};
+  int in_w = 0, in_h = 0, in_type = 0, out_type = 0, precision = 0, max_batch = 0;
+  float in_max = 1.f;
+  const __half* lut = nullptr;        // main LUT (RAVU family)
+  const __half* lut_ar = nullptr;     // zoom-ar LUT
+  mpx::NnediWeights nw{};             // NNEDI3 packed weights
+  mpx::ZoomPlan zoom{};               // zoom phase tables
+  void* scratch = nullptr;            // intermediates, max_batch frames
+  size_t scratch_bytes = 0;
+  int64_t scratch_frame = 0;          // elements per frame
+};
+

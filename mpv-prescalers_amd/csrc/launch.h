@@ -1,0 +1,17 @@
+// Host-side launchers of the HIP kernels (one per kernel family), called by abi.cpp.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace mpx {
+
+// Plane copy with element-type conversion (the output of a hook whose WHEN is false).
+hipError_t launch_copy(const Io& io, int planes, hipStream_t st);
+
+// RAVU-lite(-ar), r in {2,3,4}; lut = FP16 [288][ceil(n^2/2)][4].
+hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, hipStream_t st);
+
+}  // namespace mpx

@@ -1,0 +1,39 @@
+// Launchers of the full RAVU, RAVU-3x and RAVU-zoom kernels.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+struct mpx_ctx;
+struct mpx_plan;
+
+namespace mpx {
+
+// Per-plan phase tables of ravu-zoom (phases repeat along each axis for rational ratios).
+struct ZoomPlan {
+  float* wtab = nullptr;       // device: pre-interpolated weights per (bucket, phase pair)
+  size_t wtab_bytes = 0;
+  int32_t* xphase = nullptr;   // device: per output column, phase id
+  int32_t* yphase = nullptr;   // device: per output row, phase id
+  size_t idx_bytes = 0;
+  int nphx = 0, nphy = 0;      // distinct phases per axis (0 = per-pixel interpolation)
+};
+
+int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w, int out_h);
+void zoom_plan_destroy(mpx_ctx* ctx, mpx_plan* plan);
+
+// RAVU-3x, r in {2,3,4}; channels MPX_CH_*; lut = FP16 [216][2*ceil(n^2/2)][4].
+hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, hipStream_t st);
+
+// RAVU full: int11 into `int11` scratch (frame stride int11_frame floats), then int10/int01
+// and the interleaved 2x output.  lut = FP16 [648][n^2/8][4].
+hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
+                       hipStream_t st);
+
+// RAVU-zoom(-ar) to out_w x out_h.
+hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
+                            const ZoomPlan& zp, int out_w, int out_h, hipStream_t st);
+
+}  // namespace mpx

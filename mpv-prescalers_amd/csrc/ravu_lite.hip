@@ -1,0 +1,143 @@
+// RAVU-lite(-ar) 2x, luma: one thread per input pixel, 2x2 output block.
+//
+// Follows ravu-lite-r3.hook:22-131 (step1 hash + taps, step2 depth-to-space) and
+// ravu-lite-ar-r3.hook:90-182 (anti-ringing); the compute flavour
+// compute/ravu-lite-r3.hook:15-105 is the same math with a shared-memory tile.
+//
+// MI355X layout: a 64 x 4 pixel tile per 256-thread block (one wave per tile row, so
+// LDS reads of the window are consecutive words across lanes and conflict-free) with an
+// (r-1)-pixel clamp-to-edge halo staged once in LDS as FP32.  The FP16 LUT (rows of
+// ceil(n^2/2) RGBA texels) stays in HBM and is served from L1/L2: 13 texels x 8 B per
+// pixel for r3.  Outputs are written as two horizontal pairs per thread (rows 2y, 2y+1),
+// coalesced across the wave.
+#include "common.h"
+#include "launch.h"
+
+namespace mpx {
+
+constexpr int LITE_TX = 64;
+constexpr int LITE_TY = 4;
+
+template <int R, bool AR>
+__global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __restrict__ lut) {
+  constexpr int N = 2 * R - 1;
+  constexpr int HALO = R - 1;
+  constexpr int PW = LITE_TX + 2 * HALO;
+  constexpr int PH = LITE_TY + 2 * HALO;
+  constexpr int NTEX = (N * N + 1) / 2;
+  constexpr int C = (N * N - 1) / 2;
+  __shared__ float tile[PH * PW];
+
+  const int frame = blockIdx.z;
+  const int x0 = blockIdx.x * LITE_TX;
+  const int y0 = blockIdx.y * LITE_TY;
+  const char* inb = static_cast<const char*>(io.in);
+  const int esz_in = io.in_dt == DT_F32 ? 4 : (io.in_dt == DT_U8 ? 1 : 2);
+  const void* in = inb + static_cast<int64_t>(frame) * io.in_bstride * esz_in;
+
+  for (int i = threadIdx.x; i < PH * PW; i += blockDim.x) {
+    const int r = i / PW, c = i - r * PW;
+    const int gx = min(max(x0 + c - HALO, 0), io.W - 1);
+    const int gy = min(max(y0 + r - HALO, 0), io.H - 1);
+    tile[i] = load_elem(in, static_cast<int64_t>(gy) * io.in_pitch + gx, io.in_dt, io.in_max);
+  }
+  __syncthreads();
+
+  const int tx = threadIdx.x % LITE_TX;
+  const int ty = threadIdx.x / LITE_TX;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= io.W || y >= io.H) return;
+
+  // window sample i = X*N + Y at offset (X - HALO, Y - HALO)
+  float s[N * N];
+#pragma unroll
+  for (int X = 0; X < N; ++X)
+#pragma unroll
+    for (int Y = 0; Y < N; ++Y) s[X * N + Y] = tile[(ty + Y) * PW + tx + X];
+
+  const Bucket bk = ravu_hash<N, false, RULE_LITE>(
+      [&](int i) { return s[i]; }, [&](int i) { return static_cast<double>(s[i]); });
+  const int row = bucket_row<RULE_LITE>(bk, 288);
+  if (io.buckets)
+    io.buckets[frame * io.bucket_bstride + static_cast<int64_t>(y) * io.W + x] = row;
+
+  const __half* lrow = lut + static_cast<int64_t>(row) * NTEX * 4;
+  float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 hi = res, lo = res, hi2 = res, lo2 = res;
+#pragma unroll
+  for (int i = 0; i <= C; ++i) {
+    const float4 w = load_lut4(lrow, i);
+    const float li = s[i];
+    if (i < C) {
+      const float lj = s[N * N - 1 - i];
+      res.x += li * w.x + lj * w.w;
+      res.y += li * w.y + lj * w.z;
+      res.z += li * w.z + lj * w.y;
+      res.w += li * w.w + lj * w.x;
+    } else {
+      res.x += li * w.x;
+      res.y += li * w.y;
+      res.z += li * w.z;
+      res.w += li * w.w;
+    }
+    if (AR) {
+      const int X = i / N - HALO, Y = i % N - HALO;
+      if (X * X + Y * Y <= 4) {  // anti-ringing taps (ravu-lite-ar-r3.hook:95-181)
+        const float4 wg = make_float4(fmaxf(w.x, 0.f), fmaxf(w.y, 0.f), fmaxf(w.z, 0.f), fmaxf(w.w, 0.f));
+        float p0 = 0.1f + li, m0 = 1.1f - li;
+        float p32 = p0, m32 = m0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) { p32 *= p32; m32 *= m32; }
+        const float p33 = p32 * p0, m33 = m32 * m0;
+        hi.x += p32 * wg.x; hi.y += p32 * wg.y; hi.z += p32 * wg.z; hi.w += p32 * wg.w;
+        lo.x += m32 * wg.x; lo.y += m32 * wg.y; lo.z += m32 * wg.z; lo.w += m32 * wg.w;
+        hi2.x += p33 * wg.x; hi2.y += p33 * wg.y; hi2.z += p33 * wg.z; hi2.w += p33 * wg.w;
+        lo2.x += m33 * wg.x; lo2.y += m33 * wg.y; lo2.z += m33 * wg.z; lo2.w += m33 * wg.w;
+        if (i < C) {
+          const float lj = s[N * N - 1 - i];
+          float p1 = 0.1f + lj, m1 = 1.1f - lj;
+          float q32 = p1, n32 = m1;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) { q32 *= q32; n32 *= n32; }
+          const float q33 = q32 * p1, n33 = n32 * m1;
+          // mirrored tap uses wg.wzyx
+          hi.x += q32 * wg.w; hi.y += q32 * wg.z; hi.z += q32 * wg.y; hi.w += q32 * wg.x;
+          lo.x += n32 * wg.w; lo.y += n32 * wg.z; lo.z += n32 * wg.y; lo.w += n32 * wg.x;
+          hi2.x += q33 * wg.w; hi2.y += q33 * wg.z; hi2.z += q33 * wg.y; hi2.w += q33 * wg.x;
+          lo2.x += n33 * wg.w; lo2.y += n33 * wg.z; lo2.z += n33 * wg.y; lo2.w += n33 * wg.x;
+        }
+      }
+    }
+  }
+  float r4[4] = {res.x, res.y, res.z, res.w};
+  if (AR) {
+    const float h[4] = {hi2.x / hi.x - 0.1f, hi2.y / hi.y - 0.1f, hi2.z / hi.z - 0.1f, hi2.w / hi.w - 0.1f};
+    const float l[4] = {1.1f - lo2.x / lo.x, 1.1f - lo2.y / lo.y, 1.1f - lo2.z / lo.z, 1.1f - lo2.w / lo.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float cl = fminf(fmaxf(r4[k], l[k]), h[k]);
+      r4[k] = r4[k] * (1.0f - 0.8f) + cl * 0.8f;   // mix(res, clamp(res, lo, hi), 0.8)
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r4[k] = fminf(fmaxf(r4[k], 0.f), 1.f);
+  }
+  // res[2*dx + dy] -> output (2x+dx, 2y+dy)
+  char* outb = static_cast<char*>(io.out);
+  void* out = outb + static_cast<int64_t>(frame) * io.out_bstride * (io.out_dt == DT_F32 ? 4 : 2);
+  const int64_t o0 = static_cast<int64_t>(2 * y) * io.out_pitch + 2 * x;
+  store_pair(out, o0, io.out_dt, r4[0], r4[2]);
+  store_pair(out, o0 + io.out_pitch, io.out_dt, r4[1], r4[3]);
+}
+
+hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, hipStream_t st) {
+  dim3 grid((io.W + LITE_TX - 1) / LITE_TX, (io.H + LITE_TY - 1) / LITE_TY, io.batch);
+  dim3 block(LITE_TX * LITE_TY);
+#define MPX_LITE(R, A) \
+  if (r == R && ar == A) { hipLaunchKernelGGL((ravu_lite_kernel<R, A>), grid, block, 0, st, io, lut); return hipGetLastError(); }
+  MPX_LITE(2, false) MPX_LITE(2, true) MPX_LITE(3, false) MPX_LITE(3, true) MPX_LITE(4, false) MPX_LITE(4, true)
+#undef MPX_LITE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace mpx

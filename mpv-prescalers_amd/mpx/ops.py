@@ -1,0 +1,196 @@
+"""Python host API over the C ABI: contexts, plans and ``upscale(name, tensor)``.
+
+A variant is chosen by its reference file name without ``.hook`` (the way mpv's
+``--glsl-shader`` picks it, README.md:39-42), e.g. ``upscale("ravu-lite-ar-r3", y)``.
+Tensors are PyTorch-ROCm device tensors; their memory and stream are handed to the HIP
+kernels through ctypes.  Shapes: luma ``(H, W)`` or ``(B, H, W)``; -yuv / -rgb
+``(3, H, W)`` or ``(B, 3, H, W)`` (planar).
+"""
+import ctypes
+import threading
+
+import torch
+
+from . import _abi
+from ._abi import LIB, MpxError, MpxIo, VariantInfo, check
+
+_DT = {torch.uint8: _abi.U8, torch.uint16: _abi.U16, torch.float16: _abi.F16, torch.float32: _abi.F32}
+
+
+def variants():
+    """All shipped variant names (46), including the three ravu-zoom-ar-r3 forms whose
+    weights the reference does not ship (they report available = 0)."""
+    return [LIB.mpx_variant_name(i).decode() for i in range(LIB.mpx_variant_count())]
+
+
+def query(name, in_w, in_h, target_w, target_h):
+    """Geometry and gating (the hook's WHEN/WIDTH/HEIGHT/OFFSET) as a dict."""
+    info = VariantInfo()
+    check(LIB.mpx_query(name.encode(), in_w, in_h, target_w, target_h, ctypes.byref(info)))
+    return info.as_dict()
+
+
+def default_target(name, in_w, in_h):
+    """Target size at which a variant applies on both axes: 3x for ravu-3x, else 2x."""
+    k = 3 if name.startswith("ravu-3x") else 2
+    return in_w * k, in_h * k
+
+
+class Plan:
+    """A variant prepared for fixed sizes / element types (weights resident, scratch for
+    ``max_batch`` frames).  Not re-entrant: one plan serves one stream at a time."""
+
+    def __init__(self, ctx, name, in_w, in_h, target_w, target_h, in_dtype, out_dtype, precision="fp32",
+                 max_batch=1, bits=10):
+        self.ctx = ctx
+        self.name = name
+        self.in_w, self.in_h = in_w, in_h
+        self.in_dtype, self.out_dtype = in_dtype, out_dtype
+        self.max_batch = max_batch
+        self.precision = precision
+        h = ctypes.c_void_p()
+        rc = LIB.mpx_plan_create(ctx._h, name.encode(), in_w, in_h, target_w, target_h, _DT[in_dtype], bits,
+                                 _DT[out_dtype], _abi.PREC[precision], max_batch, ctypes.byref(h))
+        check(rc)
+        self._h = h
+        self.skipped = rc == _abi.MPX_SKIPPED
+        info = VariantInfo()
+        check(LIB.mpx_plan_info(h, ctypes.byref(info)))
+        self.info = info.as_dict()
+        self.bucket_count = LIB.mpx_plan_bucket_count(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            LIB.mpx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def out_shape(self, x):
+        lead = tuple(x.shape[:-2])
+        return lead + (self.info["out_h"], self.info["out_w"])
+
+    def run(self, x, out=None, stream=None, buckets=False):
+        """Upscale ``x`` (device tensor) into ``out`` (allocated if None).  Returns ``out``,
+        or ``(out, bucket_rows)`` with ``buckets=True``."""
+        planes = self.info["planes"]
+        if x.dtype != self.in_dtype:
+            raise TypeError("plan expects %s input, got %s" % (self.in_dtype, x.dtype))
+        if x.shape[-2:] != (self.in_h, self.in_w):
+            raise ValueError("plan expects %dx%d input, got %s" % (self.in_w, self.in_h, tuple(x.shape)))
+        if not x.is_cuda:
+            raise ValueError("input must be a device tensor")
+        nd = 2 if planes == 1 else 3
+        if x.dim() == nd:
+            batch = 1
+        elif x.dim() == nd + 1:
+            batch = x.shape[0]
+        else:
+            raise ValueError("bad input rank %d" % x.dim())
+        if planes == 3 and x.shape[-3] != 3:
+            raise ValueError("3-plane variant expects 3 planes")
+        if x.stride(-1) != 1:
+            x = x.contiguous()
+        if out is None:
+            out = torch.empty(self.out_shape(x), dtype=self.out_dtype, device=x.device)
+        if out.stride(-1) != 1 or out.dtype != self.out_dtype or tuple(out.shape) != self.out_shape(x):
+            raise ValueError("bad output tensor")
+        bk = None
+        if buckets:
+            bk = torch.empty((batch, self.bucket_count), dtype=torch.int32, device=x.device)
+        io = MpxIo()
+        io.in_ = x.data_ptr()
+        io.in_pitch = x.stride(-2)
+        io.in_cstride = x.stride(-3) if planes == 3 else x.stride(-2) * self.in_h
+        io.in_bstride = x.stride(0) if x.dim() == nd + 1 else x.numel()
+        io.out = out.data_ptr()
+        io.out_pitch = out.stride(-2)
+        io.out_cstride = out.stride(-3) if planes == 3 else out.stride(-2) * self.info["out_h"]
+        io.out_bstride = out.stride(0) if out.dim() == nd + 1 else out.numel()
+        io.batch = batch
+        io.buckets = bk.data_ptr() if bk is not None else None
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        check(LIB.mpx_run_ex(self._h, ctypes.byref(io), ctypes.c_void_p(stream.cuda_stream)))
+        return (out, bk) if buckets else out
+
+
+class Context:
+    """Device binding + resident weights + HBM budget (bytes, 0 = unlimited)."""
+
+    def __init__(self, device=0, hbm_budget=0, weights_dir=None):
+        if isinstance(device, torch.device):
+            device = device.index or 0
+        self.device = device
+        h = ctypes.c_void_p()
+        check(LIB.mpx_create(device, hbm_budget, (weights_dir or _abi.WEIGHTS_DIR).encode(), ctypes.byref(h)))
+        self._h = h
+        self._plans = {}
+        self._lock = threading.Lock()
+
+    def bytes(self):
+        return LIB.mpx_ctx_bytes(self._h)
+
+    def plan(self, name, in_w, in_h, target=None, in_dtype=torch.float32, out_dtype=torch.float32,
+             precision="fp32", max_batch=1, bits=10):
+        tw, th = target if target is not None else default_target(name, in_w, in_h)
+        return Plan(self, name, in_w, in_h, tw, th, in_dtype, out_dtype, precision, max_batch, bits)
+
+    def cached_plan(self, name, in_w, in_h, target, in_dtype, out_dtype, precision, batch, bits):
+        key = (name, in_w, in_h, target, in_dtype, out_dtype, precision, bits)
+        with self._lock:
+            p = self._plans.get(key)
+            if p is None or p.max_batch < batch:
+                if p is not None:
+                    p.close()
+                p = self.plan(name, in_w, in_h, target, in_dtype, out_dtype, precision, max(batch, 1), bits)
+                self._plans[key] = p
+            return p
+
+    def close(self):
+        for p in self._plans.values():
+            p.close()
+        self._plans.clear()
+        if getattr(self, "_h", None):
+            LIB.mpx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx = {}
+_ctx_lock = threading.Lock()
+
+
+def default_context(device=None):
+    dev = torch.cuda.current_device() if device is None else (device.index or 0 if isinstance(device, torch.device) else device)
+    with _ctx_lock:
+        if dev not in _ctx:
+            _ctx[dev] = Context(dev)
+        return _ctx[dev]
+
+
+def upscale(name, x, target=None, precision="fp32", out_dtype=None, bits=10, buckets=False, stream=None):
+    """Run variant ``name`` on device tensor ``x``.
+
+    ``target`` = (width, height) the video is displayed at (mpv's OUTPUT size): it gates
+    the hook like //!WHEN and, for ravu-zoom, is the output size.  Default: 2x (3x for
+    ravu-3x).  ``out_dtype`` defaults to the input dtype for float inputs, float32 else."""
+    planes3 = name.endswith("-yuv") or name.endswith("-rgb")
+    in_h, in_w = x.shape[-2:]
+    if out_dtype is None:
+        out_dtype = x.dtype if x.dtype in (torch.float16, torch.float32) else torch.float32
+    batch = 1
+    if x.dim() == (4 if planes3 else 3):
+        batch = x.shape[0]
+    tgt = tuple(target) if target is not None else default_target(name, in_w, in_h)
+    plan = default_context(x.device).cached_plan(name, in_w, in_h, tgt, x.dtype, out_dtype, precision, batch, bits)
+    return plan.run(x, stream=stream, buckets=buckets)

@@ -1,0 +1,49 @@
+"""Helpers shared by the GPU parity tests: run a variant through the C ABI and its FP64
+oracle on the same input, and summarise the difference."""
+import json
+import os
+import sys
+
+import numpy as np
+
+from conftest import ROOT
+
+REPORT = os.path.join(ROOT, "gpurun_out", "parity_report.jsonl")
+
+
+def psnr(err):
+    mse = float(np.mean(err ** 2))
+    return float("inf") if mse == 0 else 10 * np.log10(1.0 / mse)
+
+
+def report(case, **fields):
+    """Append one parity record (visible in the GPU run's output folder) and print it."""
+    rec = dict(case=case, **fields)
+    line = json.dumps(rec, sort_keys=True, default=float)
+    print("PARITY " + line, flush=True)
+    try:
+        os.makedirs(os.path.dirname(REPORT), exist_ok=True)
+        with open(REPORT, "a") as f:
+            f.write(line + "\n")
+    except OSError:
+        pass
+    sys.stdout.flush()
+
+
+def compare(case, got, ref, got_rows=None, ref_rows=None, site_shape=None, site_to_out=None):
+    """Max-abs / PSNR over all pixels, bucket agreement, and max-abs over pixels whose
+    producing sites agree on the bucket (site_to_out maps a site-mask to an output mask)."""
+    got = np.asarray(got, np.float64)
+    err = np.abs(got - ref)
+    fields = dict(max_abs=float(err.max()), psnr=psnr(got - ref), n_out=int(err.size))
+    if got_rows is not None:
+        gr = np.asarray(got_rows).reshape(-1)
+        rr = np.asarray(ref_rows).reshape(-1)
+        agree = gr == rr
+        fields["bucket_agree"] = float(agree.mean())
+        fields["bucket_disagree_n"] = int((~agree).sum())
+        if site_to_out is not None:
+            mask = site_to_out(agree.reshape(site_shape))
+            fields["max_abs_agreeing"] = float(err[mask].max()) if mask.any() else 0.0
+    report(case, **fields)
+    return fields
