@@ -37,9 +37,10 @@ extern "C" {
 
 /* ---- NNEDI3 contraction precision (RAVU always computes in FP32) ------------------ */
 #define MPX_PREC_FP32 0   /* plain FP32 VALU kernel                                       */
-#define MPX_PREC_FP16 1   /* FP16 MFMA, FP32 accumulation                                 */
-#define MPX_PREC_BF16 2   /* BF16 MFMA, FP32 accumulation                                 */
+#define MPX_PREC_FP16 1   /* FP16 MFMA, one product, FP32 accumulation                    */
+#define MPX_PREC_BF16 2   /* BF16 MFMA, high/low split (three products), FP32 accumulation */
 #define MPX_PREC_FP8 3    /* FP8 e4m3 MFMA, per-neuron FP32 scales                        */
+#define MPX_PREC_FP16X3 4 /* FP16 MFMA, high/low split (three products), FP32 accumulation */
 
 /* ---- variant families / channel forms --------------------------------------------- */
 #define MPX_FAMILY_NNEDI3 0

@@ -197,7 +197,7 @@ int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int targ
   if (out_type != MPX_F16 && out_type != MPX_F32) return fail(MPX_ERR_ARG, "out_type must be MPX_F16 or MPX_F32");
   if (max_batch < 1 || max_batch > 65535) return fail(MPX_ERR_SHAPE, "max_batch out of range");
   if (in_type == MPX_U16 && (bits < 1 || bits > 16)) return fail(MPX_ERR_ARG, "bits must be 1..16 for MPX_U16");
-  if (precision < MPX_PREC_FP32 || precision > MPX_PREC_FP8) return fail(MPX_ERR_ARG, "bad precision");
+  if (precision < MPX_PREC_FP32 || precision > MPX_PREC_FP16X3) return fail(MPX_ERR_ARG, "bad precision");
   if (v->family != MPX_FAMILY_NNEDI3 && precision != MPX_PREC_FP32)
     return fail(MPX_ERR_UNSUPPORTED, "RAVU variants compute in FP32 only");
 

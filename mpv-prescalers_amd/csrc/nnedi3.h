@@ -9,10 +9,20 @@ struct mpx_ctx;
 
 namespace mpx {
 
+// Device copies of one NNEDI3 network, packed for a precision.  The softmax half is
+// pre-scaled by log2(e) so the kernels use the hardware exp2.
+//   FP32:      w  = float [nns][2K+2]  rows (w1*log2e, w2, b1*log2e, b2)
+//   FP16/BF16: w  = 16-bit [nns][2][K] high parts of wscale*(w1*log2e, w2),
+//              w_lo = the same layout holding the residuals (w - high), for the
+//              three-product split; bias = float [nns][2] xscale*wscale*(b1*log2e, b2)
+//   FP8:       w  = e4m3 [nns][2][K] of w / scale; scale = float [nns][2]; bias as above
 struct NnediWeights {
   int nns = 0, wh = 0, precision = 0;
-  const void* w = nullptr;      // device table, layout depends on precision
-  const float* aux = nullptr;   // device: per-neuron FP32 terms (biases, scales, sums)
+  float xscale = 1.f, wscale = 1.f;  // power-of-two operand scales of the MFMA forms
+  const void* w = nullptr;
+  const void* w_lo = nullptr;
+  const float* bias = nullptr;
+  const float* scale = nullptr;
 };
 
 int nnedi3_prepare(mpx_ctx* ctx, int nns, int wh, int precision, NnediWeights* out);

@@ -1,0 +1,279 @@
+// RAVU (full, 2x with three predictions) and RAVU-3x, luma / yuv / rgb.
+//
+// RAVU-3x (compute/ravu-3x-r3.hook:22-163): one thread per input pixel, 3x3 output block
+// (centre = input pixel), 24x3x3 buckets, two LUT texels per tap.
+// RAVU (ravu-r3.hook:15-437): kernel 1 predicts int11 (x+1/2, y+1/2) from the 2r x 2r
+// window into an FP32 scratch plane; kernel 2 predicts int10 (x+1/2, y) and int01
+// (x, y+1/2) on the 45-degree lattice mixing the input and int11, and writes the
+// interleaved 2x output (the combine pass, :426-437).  Both read LDS tiles staged with
+// clamp-to-edge halos, each texture clamped on its own size as mpv's sampler does.
+//
+// Channel forms (ravu-r3-yuv.hook, ravu-r3-rgb.hook): the hash sees plane 0 (yuv) or
+// the BT.709 luma (rgb, `dot(rgb, vec3(0.2126, 0.7152, 0.0722))`); taps apply to all
+// three planes with the same weights.
+#include "common.h"
+#include "launch.h"
+#include "ravu.h"
+
+namespace mpx {
+
+constexpr int RT_X = 64;
+constexpr int RT_Y = 4;
+
+__device__ __forceinline__ int esz(int dt) { return dt == DT_F32 ? 4 : (dt == DT_U8 ? 1 : 2); }
+
+// Stage P planes of a frame into LDS: tile[c][r][col] = plane c at
+// (clamp(ox + col), clamp(oy + r)), PH x PW per plane.
+template <int P>
+__device__ __forceinline__ void stage_planes(float* tile, int PH, int PW, const void* base, int dt, float in_max,
+                                             int64_t pitch, int64_t cstride, int W, int H, int ox, int oy) {
+  const int n = PH * PW;
+  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
+    const int c = i / n, j = i - c * n;
+    const int r = j / PW, col = j - r * PW;
+    const int gx = min(max(ox + col, 0), W - 1);
+    const int gy = min(max(oy + r, 0), H - 1);
+    tile[i] = load_elem(base, c * cstride + static_cast<int64_t>(gy) * pitch + gx, dt, in_max);
+  }
+}
+
+// Hash-plane value from P staged planes at LDS offset o (plane stride n).
+template <int CH>
+__device__ __forceinline__ float luma_f(const float* t, int o, int n) {
+  if (CH == 2) return 0.2126f * t[o] + 0.7152f * t[n + o] + 0.0722f * t[2 * n + o];
+  return t[o];
+}
+template <int CH>
+__device__ __forceinline__ double luma_d(const float* t, int o, int n) {
+#pragma clang fp contract(off)
+  if (CH == 2)
+    return 0.2126 * static_cast<double>(t[o]) + 0.7152 * static_cast<double>(t[n + o]) +
+           0.0722 * static_cast<double>(t[2 * n + o]);
+  return static_cast<double>(t[o]);
+}
+
+// ----------------------------------------------------------------------------- RAVU-3x
+
+template <int R, int CH>
+__global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __restrict__ lut) {
+  constexpr int P = CH == 0 ? 1 : 3;
+  constexpr int N = 2 * R - 1;
+  constexpr int HALO = R - 1;
+  constexpr int PW = RT_X + 2 * HALO, PH = RT_Y + 2 * HALO, PN = PW * PH;
+  constexpr int C = (N * N - 1) / 2;
+  constexpr int NTEX = 2 * (C + 1);
+  __shared__ float tile[P * PN];
+  const int f = blockIdx.z;
+  const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
+  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz(io.in_dt);
+  stage_planes<P>(tile, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - HALO, y0 - HALO);
+  __syncthreads();
+  const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= io.W || y >= io.H) return;
+  auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };  // sample i = X*N + Y
+  const Bucket bk = ravu_hash<N, false, RULE_3X>([&](int i) { return luma_f<CH>(tile, off(i), PN); },
+                                                 [&](int i) { return luma_d<CH>(tile, off(i), PN); });
+  const int row = bucket_row<RULE_3X>(bk, 216);
+  if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * io.W + x] = row;
+  const __half* lrow = lut + static_cast<int64_t>(row) * NTEX * 4;
+  float r0[P][4], r1[P][4];
+#pragma unroll
+  for (int c = 0; c < P; ++c)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r0[c][k] = r1[c][k] = 0.f;
+#pragma unroll
+  for (int i = 0; i <= C; ++i) {
+    const float4 w0 = load_lut4(lrow, 2 * i), w1 = load_lut4(lrow, 2 * i + 1);
+    const float a0[4] = {w0.x, w0.y, w0.z, w0.w}, a1[4] = {w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      const float li = tile[c * PN + off(i)];
+      if (i < C) {
+        const float lj = tile[c * PN + off(N * N - 1 - i)];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          r0[c][k] += li * a0[k] + lj * a1[3 - k];   // res0 += l_i*w0 + l_j*w1.wzyx
+          r1[c][k] += li * a1[k] + lj * a0[3 - k];   // res1 += l_i*w1 + l_j*w0.wzyx
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          r0[c][k] += li * a0[k];
+          r1[c][k] += li * a1[k];
+        }
+      }
+    }
+  }
+  const int esz_out = io.out_dt == DT_F32 ? 4 : 2;
+  char* ob = static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * esz_out;
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    void* o = ob + c * io.out_cstride * esz_out;
+    const float centre = tile[c * PN + off(C)];
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const int k = 3 * dx + dy;  // compute/ravu-3x-r3.hook:154-162
+        float v = k < 4 ? r0[c][k] : (k == 4 ? centre : r1[c][k - 5]);
+        if (k != 4) v = fminf(fmaxf(v, 0.f), 1.f);
+        store_elem(o, static_cast<int64_t>(3 * y + dy) * io.out_pitch + 3 * x + dx, io.out_dt, v);
+      }
+  }
+}
+
+hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, hipStream_t st) {
+  dim3 grid((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
+#define MPX_3X(R, CH)                                                                                  \
+  if (r == R && channels == CH) {                                                                      \
+    hipLaunchKernelGGL((ravu_3x_kernel<R, CH>), grid, dim3(256), 0, st, io, lut);                      \
+    return hipGetLastError();                                                                          \
+  }
+  MPX_3X(2, 0) MPX_3X(2, 1) MPX_3X(2, 2) MPX_3X(3, 0) MPX_3X(3, 1) MPX_3X(3, 2) MPX_3X(4, 0) MPX_3X(4, 1) MPX_3X(4, 2)
+#undef MPX_3X
+  return hipErrorInvalidValue;
+}
+
+// ----------------------------------------------------------------------------- RAVU full
+
+// res = sum_{p < N^2/2} (s_p + s_{N^2-1-p}) * w_p, w_p = texel p/4, component p%4,
+// clamped to [0,1] (ravu-r3.hook:120-146).  s(c, i) = plane c sample i.
+template <int N, int P, typename S>
+__device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
+#pragma unroll
+  for (int c = 0; c < P; ++c) res[c] = 0.f;
+#pragma unroll
+  for (int t = 0; t < N * N / 8; ++t) {
+    const float4 w = load_lut4(lrow, t);
+    const float a[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = 4 * t + j;
+      if (p < N * N / 2) {
+#pragma unroll
+        for (int c = 0; c < P; ++c) res[c] += (s(c, p) + s(c, N * N - 1 - p)) * a[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < P; ++c) res[c] = fminf(fmaxf(res[c], 0.f), 1.f);
+}
+
+// step1: int11 from the N = 2R window at offsets -(R-1)..R (ravu-r3.hook:22-146).
+template <int R, int CH>
+__global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
+                                                         int64_t int11_frame) {
+  constexpr int P = CH == 0 ? 1 : 3;
+  constexpr int N = 2 * R;
+  constexpr int PW = RT_X + N - 1, PH = RT_Y + N - 1, PN = PW * PH;
+  __shared__ float tile[P * PN];
+  const int f = blockIdx.z;
+  const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
+  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz(io.in_dt);
+  stage_planes<P>(tile, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - (R - 1),
+                  y0 - (R - 1));
+  __syncthreads();
+  const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= io.W || y >= io.H) return;
+  auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };
+  const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(tile, off(i), PN); },
+                                                  [&](int i) { return luma_d<CH>(tile, off(i), PN); });
+  const int row = bucket_row<RULE_LOG2>(bk, 648);
+  if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * io.W + x] = row;
+  float res[P];
+  ravu_taps<N, P>(lut + static_cast<int64_t>(row) * (N * N / 8) * 4,
+                  [&](int c, int i) { return tile[c * PN + off(i)]; }, res);
+  const int64_t plane = static_cast<int64_t>(io.W) * io.H;
+  float* o = int11 + f * int11_frame + static_cast<int64_t>(y) * io.W + x;
+#pragma unroll
+  for (int c = 0; c < P; ++c) o[c * plane] = res[c];
+}
+
+// Lattice position of sample (X, Y) of step 2 (int10, centre (x+1/2, y)) / step 3
+// (int01, centre (x, y+1/2)): sample at centre + (X-(N-1)/2)*(1/2,-1/2) + (Y-(N-1)/2)*(1/2,1/2)
+// (ravu-r3.hook:155-191, 289-325).  Returns doubled coordinates.
+template <int N, int STEP>
+__device__ __host__ constexpr int lat2x(int X, int Y) { return STEP == 2 ? X + Y - N + 2 : X + Y - N + 1; }
+template <int N, int STEP>
+__device__ __host__ constexpr int lat2y(int X, int Y) { return STEP == 2 ? Y - X : Y - X + 1; }
+
+template <int R, int CH>
+__global__ __launch_bounds__(256) void ravu_step23_kernel(Io io, const __half* __restrict__ lut,
+                                                          const float* __restrict__ int11, int64_t int11_frame) {
+  constexpr int P = CH == 0 ? 1 : 3;
+  constexpr int N = 2 * R;
+  constexpr int PW = RT_X + 2 * R + 1, PH = RT_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
+  __shared__ float hk[P * PN];   // input (HOOKED)
+  __shared__ float i11[P * PN];  // ravu_int11
+  const int f = blockIdx.z;
+  const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
+  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz(io.in_dt);
+  stage_planes<P>(hk, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - R, y0 - R);
+  const int64_t plane = static_cast<int64_t>(io.W) * io.H;
+  stage_planes<P>(i11, PH, PW, int11 + f * int11_frame, DT_F32, 1.f, io.W, plane, io.W, io.H, x0 - R, y0 - R);
+  __syncthreads();
+  const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= io.W || y >= io.H) return;
+  const int base = (ty + R) * PW + tx + R;  // LDS offset of (x, y)
+  float res10[P], res01[P];
+  {
+    // step 2: int10
+    auto src = [&](int i) -> const float* {
+      const int X = i / N, Y = i % N;
+      const int px = lat2x<N, 2>(X, Y), py = lat2y<N, 2>(X, Y);
+      return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
+    };
+    const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
+                                                    [&](int i) { return luma_d<CH>(src(i), 0, PN); });
+    const int row = bucket_row<RULE_LOG2>(bk, 648);
+    if (io.buckets) io.buckets[f * io.bucket_bstride + plane + static_cast<int64_t>(y) * io.W + x] = row;
+    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * (N * N / 8) * 4, [&](int c, int i) { return src(i)[c * PN]; },
+                    res10);
+  }
+  {
+    // step 3: int01
+    auto src = [&](int i) -> const float* {
+      const int X = i / N, Y = i % N;
+      const int px = lat2x<N, 3>(X, Y), py = lat2y<N, 3>(X, Y);
+      return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
+    };
+    const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
+                                                    [&](int i) { return luma_d<CH>(src(i), 0, PN); });
+    const int row = bucket_row<RULE_LOG2>(bk, 648);
+    if (io.buckets) io.buckets[f * io.bucket_bstride + 2 * plane + static_cast<int64_t>(y) * io.W + x] = row;
+    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * (N * N / 8) * 4, [&](int c, int i) { return src(i)[c * PN]; },
+                    res01);
+  }
+  // combine (ravu-r3.hook:426-437): (2x,2y) input, (2x,2y+1) int01, (2x+1,2y) int10, (2x+1,2y+1) int11
+  const int esz_out = io.out_dt == DT_F32 ? 4 : 2;
+  char* ob = static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * esz_out;
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    void* o = ob + c * io.out_cstride * esz_out;
+    const int64_t o0 = static_cast<int64_t>(2 * y) * io.out_pitch + 2 * x;
+    store_pair(o, o0, io.out_dt, hk[c * PN + base], res10[c]);
+    store_pair(o, o0 + io.out_pitch, io.out_dt, res01[c], i11[c * PN + base]);
+  }
+}
+
+hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
+                       hipStream_t st) {
+  dim3 grid((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
+#define MPX_RAVU(R, CH)                                                                                        \
+  if (r == R && channels == CH) {                                                                              \
+    hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);       \
+    hipError_t e = hipGetLastError();                                                                          \
+    if (e != hipSuccess) return e;                                                                             \
+    hipLaunchKernelGGL((ravu_step23_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);      \
+    return hipGetLastError();                                                                                  \
+  }
+  MPX_RAVU(2, 0) MPX_RAVU(2, 1) MPX_RAVU(2, 2) MPX_RAVU(3, 0) MPX_RAVU(3, 1) MPX_RAVU(3, 2) MPX_RAVU(4, 0)
+  MPX_RAVU(4, 1) MPX_RAVU(4, 2)
+#undef MPX_RAVU
+  return hipErrorInvalidValue;
+}
+
+}  // namespace mpx

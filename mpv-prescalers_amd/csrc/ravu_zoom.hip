@@ -1,0 +1,203 @@
+// RAVU-zoom(-ar) to an arbitrary output size, luma / yuv / rgb
+// (ravu-zoom-r2.hook:15-134, ravu-zoom-ar-r2-rgb.hook:15-224, compute tiling
+// compute/ravu-zoom-r3.hook:24-45).
+//
+// Per output pixel o: pos = (o + 1/2) * in/out (OFFSET ALIGN); subpix f = fract(pos - 1/2);
+// base = pos - f.  The window is the 2r x 2r input block at offsets -(r-1)..r around base.
+// The hash depends on the base pixel only, so a block first hashes every base pixel its
+// 32 x 8 output tile touches (into LDS), then each output pixel reads its base's bucket.
+// Base and phase are computed in exact integer arithmetic:
+//   pos - 1/2 = ((2o+1)*in - out) / (2*out)  ->  base = floor, f = remainder / (2*out).
+// LUT (FILTER LINEAR): group g of 4 taps at texel x = 9g + 8f, row y = 9*bucket + 8f_y
+// (LUTPOS, :21-31), bilinear between neighbouring phase texels, rows clamped to the
+// texture; taps p < n^2/2 at phase f, tap n^2-1-p at phase 1-f.  AR variants use the
+// separate learned ``_ar`` LUT with the same addressing (all taps), per channel, strength
+// 0.8, no final clamp.  The LUTs (0.37 / 0.93 MB FP16) stay in L2.
+#include "common.h"
+#include "launch.h"
+#include "ravu.h"
+
+namespace mpx {
+
+constexpr int ZT_X = 32;
+constexpr int ZT_Y = 8;
+constexpr int ZB_X = ZT_X + 1;  // base pixels an output tile can touch (ratio in/out < 1)
+constexpr int ZB_Y = ZT_Y + 1;
+
+__device__ __forceinline__ int floor_div(int64_t a, int64_t b) {  // b > 0
+  int64_t q = a / b;
+  if ((a % b) != 0 && (a < 0)) --q;
+  return static_cast<int>(q);
+}
+
+struct Phase {
+  int base;
+  float f;
+};
+__device__ __forceinline__ Phase phase_of(int o, int in, int out) {
+  const int64_t num = static_cast<int64_t>(2 * o + 1) * in - out;
+  const int64_t den = 2 * static_cast<int64_t>(out);
+  Phase p;
+  p.base = floor_div(num, den);
+  p.f = static_cast<float>(num - static_cast<int64_t>(p.base) * den) / static_cast<float>(den);
+  return p;
+}
+
+// FILTER LINEAR fetch of group g at texel coordinates (9g + tx, ty), tx in [0, 8],
+// ty unclamped; clamp-to-edge on the texture.
+__device__ __forceinline__ float4 lut_bilinear(const __half* lut, int width, int rows, int g, float tx, float ty) {
+  const float fx0 = floorf(tx), fy0 = floorf(ty);
+  const float ax = tx - fx0, ay = ty - fy0;
+  const int x0 = 9 * g + static_cast<int>(fx0);
+  const int x1 = min(x0 + 1, width - 1);
+  const int y0i = static_cast<int>(fy0);
+  const int y0 = min(max(y0i, 0), rows - 1), y1 = min(max(y0i + 1, 0), rows - 1);
+  const float4 v00 = load_lut4(lut, static_cast<int64_t>(y0) * width + x0);
+  const float4 v01 = load_lut4(lut, static_cast<int64_t>(y0) * width + x1);
+  const float4 v10 = load_lut4(lut, static_cast<int64_t>(y1) * width + x0);
+  const float4 v11 = load_lut4(lut, static_cast<int64_t>(y1) * width + x1);
+  const float bx = 1.f - ax, by = 1.f - ay;
+  float4 r;
+  r.x = (v00.x * bx + v01.x * ax) * by + (v10.x * bx + v11.x * ax) * ay;
+  r.y = (v00.y * bx + v01.y * ax) * by + (v10.y * bx + v11.y * ax) * ay;
+  r.z = (v00.z * bx + v01.z * ax) * by + (v10.z * bx + v11.z * ax) * ay;
+  r.w = (v00.w * bx + v01.w * ax) * by + (v10.w * bx + v11.w * ax) * ay;
+  return r;
+}
+
+template <int CH>
+__device__ __forceinline__ float zluma_f(const float* t, int o, int n) {
+  if (CH == 2) return 0.2126f * t[o] + 0.7152f * t[n + o] + 0.0722f * t[2 * n + o];
+  return t[o];
+}
+template <int CH>
+__device__ __forceinline__ double zluma_d(const float* t, int o, int n) {
+#pragma clang fp contract(off)
+  if (CH == 2)
+    return 0.2126 * static_cast<double>(t[o]) + 0.7152 * static_cast<double>(t[n + o]) +
+           0.0722 * static_cast<double>(t[2 * n + o]);
+  return static_cast<double>(t[o]);
+}
+
+template <int R, bool AR, int CH>
+__global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
+                                                        const __half* __restrict__ lut_ar, int out_w, int out_h) {
+  constexpr int P = CH == 0 ? 1 : 3;
+  constexpr int N = 2 * R;
+  constexpr int HALF = N * N / 2;
+  constexpr int GROUPS = (HALF + 3) / 4;
+  constexpr int WIDTH = 9 * GROUPS;      // LUT texels per row
+  constexpr int ROWS = 2592;             // 288 buckets x 9 phases
+  constexpr int PW = ZB_X + N - 1, PH = ZB_Y + N - 1, PN = PW * PH;
+  __shared__ float tile[P * PN];
+  __shared__ int bucket[ZB_Y * ZB_X];
+
+  const int f = blockIdx.z;
+  const int ox0 = blockIdx.x * ZT_X, oy0 = blockIdx.y * ZT_Y;
+  const int bx0 = phase_of(ox0, io.W, out_w).base;
+  const int by0 = phase_of(oy0, io.H, out_h).base;
+  const int esz_in = io.in_dt == DT_F32 ? 4 : (io.in_dt == DT_U8 ? 1 : 2);
+  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz_in;
+  for (int i = threadIdx.x; i < P * PN; i += blockDim.x) {
+    const int c = i / PN, j = i - c * PN;
+    const int r = j / PW, col = j - r * PW;
+    const int gx = min(max(bx0 - (R - 1) + col, 0), io.W - 1);
+    const int gy = min(max(by0 - (R - 1) + r, 0), io.H - 1);
+    tile[i] = load_elem(in, c * io.in_cstride + static_cast<int64_t>(gy) * io.in_pitch + gx, io.in_dt, io.in_max);
+  }
+  __syncthreads();
+  // hash every base pixel of the tile (unclamped LUT row: bucket 9-row blocks are clamped
+  // per texel by the bilinear fetch, as the sampler does)
+  for (int i = threadIdx.x; i < ZB_Y * ZB_X; i += blockDim.x) {
+    const int by = i / ZB_X, bx = i - by * ZB_X;
+    auto off = [&](int s) { return (by + s % N) * PW + bx + s / N; };
+    const Bucket bk = ravu_hash<N, true, RULE_LITE>([&](int s) { return zluma_f<CH>(tile, off(s), PN); },
+                                                    [&](int s) { return zluma_d<CH>(tile, off(s), PN); });
+    bucket[i] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+  }
+  __syncthreads();
+
+  const int ox = ox0 + threadIdx.x % ZT_X, oy = oy0 + threadIdx.x / ZT_X;
+  if (ox >= out_w || oy >= out_h) return;
+  const Phase phx = phase_of(ox, io.W, out_w), phy = phase_of(oy, io.H, out_h);
+  const int lbx = phx.base - bx0, lby = phy.base - by0;  // in [0, ZB) by construction
+  const int row = bucket[lby * ZB_X + lbx];
+  if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(oy) * out_w + ox] = row;
+  auto s = [&](int c, int i) { return tile[c * PN + (lby + i % N) * PW + lbx + i / N]; };
+  const float txf = 8.f * phx.f, txi = 8.f * (1.f - phx.f);
+  const float tyf = 9.f * row + 8.f * phy.f, tyi = 9.f * row + 8.f * (1.f - phy.f);
+  float res[P], hi[P], lo[P], hi2[P], lo2[P];
+#pragma unroll
+  for (int c = 0; c < P; ++c) res[c] = hi[c] = lo[c] = hi2[c] = lo2[c] = 0.f;
+#pragma unroll
+  for (int g = 0; g < GROUPS; ++g) {
+    const float4 wf4 = lut_bilinear(lut, WIDTH, ROWS, g, txf, tyf);
+    const float4 wi4 = lut_bilinear(lut, WIDTH, ROWS, g, txi, tyi);
+    const float wf[4] = {wf4.x, wf4.y, wf4.z, wf4.w}, wi[4] = {wi4.x, wi4.y, wi4.z, wi4.w};
+    float af[4] = {0.f, 0.f, 0.f, 0.f}, ai[4] = {0.f, 0.f, 0.f, 0.f};
+    if (AR) {
+      const float4 a4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txf, tyf);
+      const float4 b4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txi, tyi);
+      af[0] = a4.x; af[1] = a4.y; af[2] = a4.z; af[3] = a4.w;
+      ai[0] = b4.x; ai[1] = b4.y; ai[2] = b4.z; ai[3] = b4.w;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = 4 * g + k;
+      if (p < HALF) {
+        const int q = N * N - 1 - p;
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          const float lp = s(c, p), lq = s(c, q);
+          res[c] += lp * wf[k] + lq * wi[k];
+          if (AR) {
+            // ravu-zoom-ar-r2-rgb.hook:152-222: (0.1+l)^32, (1.1-l)^32 and the ^33 terms
+            float p0 = 0.1f + lp, m0 = 1.1f - lp, p1 = 0.1f + lq, m1 = 1.1f - lq;
+            float p32 = p0, m32 = m0, q32 = p1, n32 = m1;
+#pragma unroll
+            for (int e = 0; e < 5; ++e) { p32 *= p32; m32 *= m32; q32 *= q32; n32 *= n32; }
+            hi[c] += p32 * af[k] + q32 * ai[k];
+            lo[c] += m32 * af[k] + n32 * ai[k];
+            hi2[c] += p32 * p0 * af[k] + q32 * p1 * ai[k];
+            lo2[c] += m32 * m0 * af[k] + n32 * m1 * ai[k];
+          }
+        }
+      }
+    }
+  }
+  const int esz_out = io.out_dt == DT_F32 ? 4 : 2;
+  char* ob = static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * esz_out;
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    float v = res[c];
+    if (AR) {
+      const float h = hi2[c] / hi[c] - 0.1f, l = 1.1f - lo2[c] / lo[c];
+      const float cl = fminf(fmaxf(v, l), h);
+      v = v * (1.0f - 0.8f) + cl * 0.8f;  // mix(res, clamp(res, lo, hi), 0.8), no final clamp
+    } else {
+      v = fminf(fmaxf(v, 0.f), 1.f);
+    }
+    store_elem(ob + c * io.out_cstride * esz_out, static_cast<int64_t>(oy) * io.out_pitch + ox, io.out_dt, v);
+  }
+}
+
+hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
+                            const ZoomPlan& zp, int out_w, int out_h, hipStream_t st) {
+  (void)zp;
+  dim3 grid((out_w + ZT_X - 1) / ZT_X, (out_h + ZT_Y - 1) / ZT_Y, io.batch);
+#define MPX_ZOOM(R, A, CH)                                                                             \
+  if (r == R && ar == A && channels == CH) {                                                           \
+    hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH>), grid, dim3(256), 0, st, io, lut, lut_ar, out_w, out_h); \
+    return hipGetLastError();                                                                          \
+  }
+  MPX_ZOOM(2, false, 0) MPX_ZOOM(2, false, 1) MPX_ZOOM(2, false, 2) MPX_ZOOM(3, false, 0) MPX_ZOOM(3, false, 1)
+  MPX_ZOOM(3, false, 2) MPX_ZOOM(2, true, 0) MPX_ZOOM(2, true, 1) MPX_ZOOM(2, true, 2) MPX_ZOOM(3, true, 0)
+  MPX_ZOOM(3, true, 1) MPX_ZOOM(3, true, 2)
+#undef MPX_ZOOM
+  return hipErrorInvalidValue;
+}
+
+int zoom_plan_create(mpx_ctx*, mpx_plan*, int, int, int, int) { return 0; }
+void zoom_plan_destroy(mpx_ctx*, mpx_plan*) {}
+
+}  // namespace mpx

@@ -16,7 +16,7 @@ STATUS = {
     -5: "MPX_ERR_BUDGET", -6: "MPX_ERR_HIP", -7: "MPX_ERR_IO", -8: "MPX_ERR_UNSUPPORTED",
 }
 U8, U16, F16, F32 = 0, 1, 2, 3
-PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "fp8": 3}
+PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "fp8": 3, "fp16x3": 4}
 FAMILY = {0: "nnedi3", 1: "ravu", 2: "ravu-lite", 3: "ravu-3x", 4: "ravu-zoom"}
 CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 

@@ -249,7 +249,8 @@ def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None):
     texel x = 9g + 8f, y = 9*bucket + 8f_y (LUTPOS(f) = (1/2+8f)/9 in normalised coords);
     taps p < n^2/2 use phase f, tap n^2-1-p uses phase 1-f with the same (group, comp).
     Non-AR clamps to [0,1]; AR uses the separate learned `_ar` LUT (all taps, weights as
-    stored), per channel, strength 0.8, no final clamp."""
+    stored), per channel, strength 0.8, no final clamp.  Base and phase follow from the
+    exact rational pos - 1/2 = ((2o+1)*in - out) / (2*out)."""
     planes = _as_planes(img, channels)
     C, H, Wd = planes.shape
     n = 2 * r
@@ -259,12 +260,12 @@ def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None):
             ar_lut = load_lut16("ravu_zoom_lut%d_ar" % r)
     ox = np.arange(out_w)
     oy = np.arange(out_h)
-    px = (ox + 0.5) * Wd / out_w
-    py = (oy + 0.5) * H / out_h
-    fx = (px - 0.5) - np.floor(px - 0.5)
-    fy = (py - 0.5) - np.floor(py - 0.5)
-    bx = np.floor(px - 0.5).astype(np.int64)            # base texel index
-    by = np.floor(py - 0.5).astype(np.int64)
+    numx = (2 * ox + 1) * Wd - out_w
+    numy = (2 * oy + 1) * H - out_h
+    bx = numx // (2 * out_w)                            # base texel index (floor)
+    by = numy // (2 * out_h)
+    fx = (numx - bx * 2 * out_w) / (2.0 * out_w)        # subpix = fract(pos - 0.5)
+    fy = (numy - by * 2 * out_h) / (2.0 * out_h)
     lo = -(r - 1)
     samp = []
     for X in range(n):
@@ -278,7 +279,9 @@ def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None):
         lum = [luma_of(s, channels) for s in samp]
     a, b, d = structure_tensor(lum, n, five_point=True)
     angle, strength, coherence, diag = hash_abd(a, b, d, STRENGTH_LITE)
-    bucket = lut_row(angle, strength, coherence, 4, 288)   # (out_h, out_w)
+    # unclamped bucket (angle can be -1 or 24 at the mod wrap): the LINEAR fetch clamps
+    # texel rows to the texture, not the bucket index
+    bucket = ((angle * 4 + strength) * 3 + coherence).astype(np.int64)   # (out_h, out_w)
     FX = np.broadcast_to(fx[None, :], (out_h, out_w))
     FY = np.broadcast_to(fy[:, None], (out_h, out_w))
     ty = 9 * bucket + 8 * FY

@@ -1,0 +1,94 @@
+"""NNEDI3 on the GPU vs the FP64 oracle (nnedi3-nns*-win8x*.hook).
+
+Bars (max-abs vs the FP64 oracle): plain FP32 VALU form 2e-5; FP16 MFMA single product
+(FP32 accumulation) 2^-10, the target stated for the FP16 form; the three-product
+high/low splits 2e-5 (FP16, operands pre-scaled by 2^8) and 5e-5 (BF16).  Kept
+rows/columns of the combine passes are exact copies of the input."""
+import numpy as np
+import pytest
+import torch
+
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 2e-5, "fp16": 2.0 ** -10, "fp16x3": 2e-5, "bf16": 5e-5}
+TOL_MFMA = TOL["fp16"]
+
+
+def _frame(w, h, seed, bits=8):
+    from mpx.frames import make_plane
+    return make_plane(w, h, seed, bits).astype(np.float32)
+
+
+@pytest.mark.parametrize("nns", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("wh", [4, 6])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16x3", "bf16"])
+def test_nnedi3_parity_small(nns, wh, prec):
+    import mpx
+    from oracle.nnedi3 import nnedi3
+    img = _frame(200, 120, seed=nns + wh)
+    out = mpx.upscale("nnedi3-nns%d-win8x%d" % (nns, wh), torch.from_numpy(img).cuda(), precision=prec)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    ref = nnedi3(img.astype(np.float64), nns, wh)
+    f = compare("nnedi3-nns%d-win8x%d %s 200x120" % (nns, wh, prec), got, ref)
+    assert f["max_abs"] <= TOL[prec]
+    # combine passes: (2x, 2y) is the input pixel, bit-exact
+    assert np.array_equal(got[0::2, 0::2], img)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16x3"])
+def test_nnedi3_nns32_1080p_fp16_input(prec):
+    """The 1920x1080 -> 3840x2160 workload: fp16 input plane, fp16 output."""
+    import mpx
+    from oracle.nnedi3 import nnedi3
+    img = _frame(1920, 1080, seed=11).astype(np.float16)
+    out = mpx.upscale("nnedi3-nns32-win8x4", torch.from_numpy(img).cuda(), precision=prec,
+                      out_dtype=torch.float32)
+    torch.cuda.synchronize()
+    ref = nnedi3(img.astype(np.float64), 32, 4)
+    f = compare("nnedi3-nns32-win8x4 %s 1920x1080 f16-in" % prec, out.cpu().numpy(), ref)
+    assert f["max_abs"] <= TOL[prec]
+
+
+@pytest.mark.parametrize("axes_target", [((64, 96), "y"), ((128, 48), "x")])
+def test_nnedi3_single_axis(axes_target):
+    """WHEN gates each axis separately: only the doubled axis is processed."""
+    import mpx
+    from oracle.nnedi3 import nnedi3
+    target, axes = axes_target
+    img = _frame(64, 48, seed=5)
+    out = mpx.upscale("nnedi3-nns64-win8x6", torch.from_numpy(img).cuda(), target=target, precision="fp16")
+    ref = nnedi3(img.astype(np.float64), 64, 6, axes=axes)
+    assert out.shape == ref.shape
+    assert np.abs(out.cpu().numpy() - ref).max() <= TOL_MFMA
+
+
+def test_nnedi3_batch_u16_10bit():
+    """Batched 10-bit frames in 16-bit containers: each equals its own float run."""
+    import mpx
+    frames = np.stack([_frame(80, 56, seed=s, bits=10) for s in range(3)])
+    u16 = np.round(frames * 1023).astype(np.uint16)
+    x = torch.from_numpy(u16.astype(np.int32)).to(torch.uint16).cuda()
+    out = mpx.upscale("nnedi3-nns64-win8x6", x, precision="fp16", bits=10)
+    for i in range(3):
+        xf = torch.from_numpy(u16[i].astype(np.float32) / 1023.0).cuda()
+        single = mpx.upscale("nnedi3-nns64-win8x6", xf, precision="fp16")
+        assert torch.equal(out[i], single), i
+
+
+def test_nnedi3_flat_and_tiny():
+    """Constant frame: zero variance, the prediction is the window mean (exactly the
+    constant); 1x1 and 3x2 frames clamp everywhere."""
+    import mpx
+    from oracle.nnedi3 import nnedi3
+    c = np.full((40, 70), 0.375, np.float32)
+    out = mpx.upscale("nnedi3-nns16-win8x4", torch.from_numpy(c).cuda(), precision="fp16").cpu().numpy()
+    assert np.all(out == np.float32(0.375))
+    for (w, h) in [(1, 1), (3, 2), (9, 5)]:
+        img = _frame(w, h, seed=w)
+        for prec in ("fp32", "fp16"):
+            out = mpx.upscale("nnedi3-nns32-win8x6", torch.from_numpy(img).cuda(), precision=prec).cpu().numpy()
+            ref = nnedi3(img.astype(np.float64), 32, 6)
+            assert np.abs(out - ref).max() <= TOL_MFMA, (w, h, prec)

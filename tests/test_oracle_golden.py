@@ -1,0 +1,142 @@
+"""The FP64 oracle against golden vectors produced by executing the reference's own
+`.hook` text (tests/golden/make_golden.py -> hooks_golden.npz), for all 43 shipped
+variants, plus closed-form checks the mathematics fixes.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+GOLD = os.path.join(ROOT, "tests", "golden", "hooks_golden.npz")
+_G = np.load(GOLD) if os.path.exists(GOLD) else None
+NAMES = sorted({k.split("/")[0] for k in _G.files}) if _G is not None else []
+
+
+def run_oracle(name, img, target):
+    from oracle.nnedi3 import nnedi3
+    from oracle.ravu import ravu, ravu_3x, ravu_lite, ravu_zoom
+    ch = "yuv" if name.endswith("-yuv") else ("rgb" if name.endswith("-rgb") else "luma")
+    if name.startswith("nnedi3"):
+        nns = int(name.split("nns")[1].split("-")[0])
+        return nnedi3(img, nns, int(name[-1]))
+    import re
+    r = int(re.search(r"-r(\d)", name).group(1))
+    if name.startswith("ravu-lite"):
+        return ravu_lite(img, r, ar="-ar-" in name)[0]
+    if name.startswith("ravu-3x"):
+        return ravu_3x(img, r, channels=ch)[0]
+    if name.startswith("ravu-zoom"):
+        return ravu_zoom(img, r, int(target[0]), int(target[1]), ar="-ar-" in name, channels=ch)[0]
+    return ravu(img, r, channels=ch)[0]
+
+
+def test_golden_covers_every_shipped_variant():
+    import mpx
+    shipped = [n for n in mpx.variants() if mpx.query(n, 8, 8, 16, 16)["available"]]
+    assert sorted(shipped) == NAMES
+    assert len(NAMES) == 43
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_matches_reference_text(name):
+    img = _G[name + "/in"]
+    tgt = _G[name + "/target"]
+    want = _G[name + "/out"]
+    got = run_oracle(name, img, tgt)
+    assert got.shape == want.shape
+    # both are FP64 evaluations of the same formulas; only summation order and the
+    # sampler's coordinate arithmetic differ
+    tol = 1e-12 if name.startswith("ravu-zoom") else 1e-13
+    assert np.abs(got - want).max() <= tol
+
+
+def test_goldens_reproduce_from_reference_text(reference_dir):
+    """The committed fixtures are what the executor produces from the reference now."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import hookexec
+    for name in ("ravu-lite-ar-r3", "nnedi3-nns32-win8x4", "ravu-zoom-ar-r2-rgb"):
+        out = hookexec.run_hook(os.path.join(reference_dir, name + ".hook"), _G[name + "/in"],
+                                tuple(_G[name + "/target"]))
+        assert np.array_equal(out, _G[name + "/out"]), name
+
+
+# ----------------------------------------------------------------------------- closed forms
+
+def test_gaussian_weights_reproduce_shader_literals(reference_dir):
+    """The closed-form stencil weights equal the FP64 literals of every main pass."""
+    import re
+    from oracle.common import gaussian_grid
+    for f, n in (("ravu-lite-r2.hook", 3), ("ravu-lite-r3.hook", 5), ("ravu-lite-r4.hook", 7), ("ravu-r2.hook", 4),
+                 ("ravu-r3.hook", 6), ("ravu-r4.hook", 8), ("ravu-zoom-r3.hook", 6), ("compute/ravu-3x-r4.hook", 7)):
+        first = open(os.path.join(reference_dir, f)).read().split("//!DESC")[1]
+        lits = [float(x) for x in re.findall(r"abd \+= vec3\(gx \* gx, gx \* gy, gy \* gy\) \* ([0-9.e-]+);", first)]
+        assert lits == gaussian_grid(n)[1], f
+
+
+def test_generated_gauss_header_matches_closed_form():
+    import re
+    from oracle.common import gaussian_grid
+    txt = open(os.path.join(ROOT, "mpv-prescalers_amd", "csrc", "gauss_tables.h")).read()
+    for n in range(3, 9):
+        body = re.search(r"kGauss%d\[\d+\] = \{\s*([^}]*)\}" % n, txt).group(1)
+        vals = [float.fromhex(v.strip()) for v in body.split(",")]
+        assert vals == gaussian_grid(n)[1]
+
+
+@pytest.mark.parametrize("fn", ["lite", "3x", "ravu", "zoom", "nnedi3"])
+def test_flat_frame_maps_to_itself(fn):
+    """A constant frame: every gradient is 0 (bucket 0 row); RAVU outputs are the constant
+    times the FP16-rounded weight sum (within 2e-4 of it); NNEDI3 (zero variance) returns
+    the window mean exactly."""
+    from oracle.nnedi3 import nnedi3
+    from oracle.ravu import ravu, ravu_3x, ravu_lite, ravu_zoom
+    c = np.full((12, 16), 0.4375)
+    if fn == "nnedi3":
+        assert np.array_equal(nnedi3(c, 64, 6), np.full((24, 32), 0.4375))
+        return
+    out, aux = {"lite": lambda: ravu_lite(c, 3), "3x": lambda: ravu_3x(c, 3), "ravu": lambda: ravu(c, 3),
+                "zoom": lambda: ravu_zoom(c, 2, 24, 18)}[fn]()
+    assert np.abs(out - 0.4375).max() < 2e-4
+    rows = aux["row"]["int11"] if isinstance(aux["row"], dict) else aux["row"]
+    assert np.all(rows == 0)
+
+
+def test_edge_angle_bucket():
+    """A straight step edge whose gradient points at angle phi: the dominant eigenvector
+    (b, L1-a) of the structure tensor is the gradient direction, so theta = mod(phi, pi)
+    up to the window's discretisation (within one of the 24 angle buckets)."""
+    from oracle.common import PI, STRENGTH_LITE, hash_abd, structure_tensor, tex_off
+    for phi in (0.3, 1.1, 2.0, 2.8):
+        y, x = np.mgrid[0:31, 0:31].astype(np.float64)
+        d = (x - 15) * np.cos(phi) + (y - 15) * np.sin(phi)
+        img = 0.2 + 0.5 * np.clip(d / 2 + 0.5, 0, 1)
+        samp = [tex_off(img, X - 2, Y - 2) for X in range(5) for Y in range(5)]
+        a, b, dd = structure_tensor(samp, 5, False)
+        ang, _, coh, diag = hash_abd(a, b, dd, STRENGTH_LITE)
+        theta = diag["theta"][15, 15]
+        want = phi % PI
+        err = min(abs(theta - want), PI - abs(theta - want))
+        assert err < PI / 24, (phi, theta, want)
+        assert coh[15, 15] == 2  # a single edge is fully coherent
+
+
+def test_ar_constant_taps():
+    """Anti-ringing over constant taps: lo = hi = c, so res becomes 0.2*res + 0.8*c."""
+    from oracle.ravu import _ar_clamp
+    c = 0.3
+    res = np.array([0.9, 0.1, 0.3])
+    taps = [np.full(3, c)] * 5
+    w = [np.full(3, 0.2)] * 5
+    out = _ar_clamp(res, taps, w)
+    assert np.allclose(out, 0.2 * res + 0.8 * c, atol=1e-12)
+
+
+def test_nnedi3_transpose_symmetry():
+    """double_x is double_y on the transposed plane (the shipped x-pass weights are the
+    exact transpose of the y-pass weights)."""
+    from mpx.frames import make_plane
+    from oracle.nnedi3 import double_x, double_y
+    img = make_plane(20, 14, 9)
+    assert np.array_equal(double_x(img, 32, 6), double_y(img.T.copy(), 32, 6).T)
