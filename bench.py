@@ -55,22 +55,121 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))]
 
 
-def cpu_baseline(img16, budget_s=20.0):
-    """FP64 NumPy oracle on the host: whole 1080p frames (both passes) until ~budget_s."""
+def cpu_baseline(img16, budget_s=12.0):
+    """The oracle's C restatement (FP32, OpenMP over all host threads) on whole 1080p
+    frames of the same workload, until ~budget_s of CPU time."""
     import numpy as np
-    from oracle.nnedi3 import nnedi3
-    x = img16.astype(np.float64)
+    from oracle import cbuild
+    x = img16.astype(np.float32)
+    cbuild.nnedi3(x[:64, :64], 32, 4)  # load + warm
     n, t0 = 0, time.perf_counter()
     while True:
-        nnedi3(x, 32, 4)
+        cbuild.nnedi3(x, 32, 4)
         n += 1
         dt = time.perf_counter() - t0
-        if dt >= budget_s or n >= 3:
+        if dt >= budget_s or n >= 40:
             break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(n * 4 * W * H / dt / 1e6, 3), "unit": "Mpixel/s", "cores": cores, "kind": "port",
-            "sample": "%d full 1920x1080 frame(s), nnedi3-nns32-win8x4, FP64 NumPy oracle (BLAS threads=%d), %.1f s"
-                      % (n, cores, dt)}
+            "sample": "%d full 1920x1080 frame(s) of nnedi3-nns32-win8x4, C FP32 OpenMP restatement "
+                      "(oracle/c/mpx_cpu.c), %.1f s" % (n, dt)}
+
+
+SERVE_VARIANTS = ("ravu-lite-ar-r3", "ravu-r4-yuv", "ravu-3x-r3", "nnedi3-nns64-win8x6")
+SERVE_SIZES = ((640, 360), (1280, 720), (1366, 768), (1920, 1080), (2560, 1440))
+
+
+def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
+    """Poisson stream of 10-bit FP16 frames, sizes and variants uniform over the sets
+    above, through the continuous-batching loop: p50/p99 latency (arrival -> result
+    ready) at the offered load, the loop's saturated throughput, and the throughput of
+    one frame per launch on one stream for the same requests."""
+    import numpy as np
+    import torch
+    from mpx.frames import make_frame
+    from mpx.serving import Server, percentile
+    dev = torch.device("cuda:0")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pool = {}
+    for v in SERVE_VARIANTS:
+        planes = 3 if v.endswith("-yuv") else 1
+        for (w, h) in SERVE_SIZES:
+            pool[(v, w, h)] = [torch.from_numpy(make_frame(w, h, 7 + i, bits=10, channels=planes)
+                                                .astype(np.float16)).to(dev) for i in range(2)]
+    prec = {v: ("fp16" if v.startswith("nnedi3") else "fp32") for v in SERVE_VARIANTS}
+    n = int(duration_s * rate)
+    gaps = rng.exponential(1.0 / rate, size=n)
+    pick_v = rng.integers(0, len(SERVE_VARIANTS), size=n)
+    pick_s = rng.integers(0, len(SERVE_SIZES), size=n)
+    reqs = []
+    for i in range(n):
+        v = SERVE_VARIANTS[pick_v[i]]
+        w, h = SERVE_SIZES[pick_s[i]]
+        reqs.append((v, pool[(v, w, h)][i % 2]))
+
+    def out_px(v, x):
+        k = 3 if v.startswith("ravu-3x") else 2
+        return k * k * x.shape[-1] * x.shape[-2] * (3 if v.endswith("-yuv") else 1)
+    total_px = sum(out_px(v, x) for v, x in reqs)
+
+    srv = Server(device=0, hbm_budget=16 << 30, streams=4, max_batch=8)
+    try:
+        for key, frames in pool.items():   # plans + weights resident before timing
+            srv.submit(key[0], frames[0], precision=prec[key[0]]).wait(120)
+        torch.cuda.synchronize()
+        # (a) Poisson arrivals at `rate`
+        t0 = time.perf_counter()
+        arr = t0 + np.cumsum(gaps)
+        handles = []
+        for i, (v, x) in enumerate(reqs):
+            d = arr[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            handles.append(srv.submit(v, x, precision=prec[v]))
+        for r in handles:
+            r.wait(300)
+        t_end = max(r.t_done for r in handles)
+        lat = [r.latency * 1e3 for r in handles]
+        offered = {"p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+                   "mpx_s": round(total_px / (t_end - t0) / 1e6, 1), "frames": n,
+                   "achieved_fps": round(n / (t_end - t0), 1)}
+        # (b) saturated: everything queued at once
+        t0 = time.perf_counter()
+        handles = [srv.submit(v, x, precision=prec[v]) for v, x in reqs]
+        for r in handles:
+            r.wait(300)
+        t_end = max(r.t_done for r in handles)
+        sat = total_px / (t_end - t0) / 1e6
+        launches = srv.launches
+    finally:
+        srv.close()
+    # (c) one frame per launch on one stream: the plain request loop (launch, wait for the
+    # result, next request) and, for reference, the same launches issued back to back
+    import mpx
+    ctx = mpx.Context(0)
+    plans = {}
+    for key, frames in pool.items():
+        plans[key] = ctx.plan(key[0], key[1], key[2], in_dtype=torch.float16, out_dtype=torch.float16,
+                              precision=prec[key[0]])
+        plans[key].run(frames[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for v, x in reqs:
+        plans[(v, x.shape[-1], x.shape[-2])].run(x)
+        torch.cuda.synchronize()
+    single_sync = total_px / (time.perf_counter() - t0) / 1e6
+    t0 = time.perf_counter()
+    for v, x in reqs:
+        plans[(v, x.shape[-1], x.shape[-2])].run(x)
+    torch.cuda.synchronize()
+    single_async = total_px / (time.perf_counter() - t0) / 1e6
+    return {"workload": "Poisson %.0f frames/s, sizes %s, variants %s, 10-bit fp16 frames" %
+                        (rate, "/".join("%dx%d" % s_ for s_ in SERVE_SIZES), "/".join(SERVE_VARIANTS)),
+            "offered_load": offered, "saturated_mpx_s": round(sat, 1),
+            "one_frame_per_launch_sync_mpx_s": round(single_sync, 1),
+            "one_frame_per_launch_async_mpx_s": round(single_async, 1),
+            "speedup_vs_one_frame_per_launch_sync": round(sat / single_sync, 2),
+            "speedup_vs_one_frame_per_launch_async": round(sat / single_async, 2), "launches": launches}
 
 
 def main():
@@ -80,6 +179,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (frames shard trivially across "
@@ -154,6 +254,10 @@ def main():
                 "mpx_s": round(b * 10 * opx / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
                 "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1)}
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_bench()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(img)
@@ -181,6 +285,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "secondary": secondary,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0

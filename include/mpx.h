@@ -133,6 +133,15 @@ int mpx_run(mpx_plan* plan, const void* in, int64_t in_pitch, int64_t in_bstride
             int64_t out_pitch, int64_t out_bstride, int batch, void* hip_stream);
 int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream);
 
+/* Run the plan on n (1..MPX_MAX_FRAMES, <= max_batch) frames of the plan's shape that live
+ * at independent addresses (ins[i] / outs[i]: host arrays of device pointers; planes of
+ * a frame contiguous).  One launch per pass for all of them, no copies and no
+ * allocation -- what a serving loop uses to batch frames of separate requests.  Replaces
+ * mpv dispatching the hook's passes once per frame. */
+#define MPX_MAX_FRAMES 16
+int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* outs, int64_t in_pitch,
+                   int64_t out_pitch, void* hip_stream);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* mpx_last_error(void);
 

@@ -274,6 +274,8 @@ void mpx_plan_destroy(mpx_plan* plan) {
   delete plan;
 }
 
+static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st);
+
 int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
   if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
   if (!io->in || !io->out) return fail(MPX_ERR_ARG, "in/out is NULL");
@@ -291,6 +293,7 @@ int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
   HIP_TRY(hipSetDevice(plan->ctx->device));
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
   mpx::Io d{};
+  d.ntab = 0;
   d.in = io->in;
   d.out = io->out;
   d.in_pitch = io->in_pitch;
@@ -307,6 +310,43 @@ int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = io->batch;
+  return dispatch(plan, d, st);
+}
+
+int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* outs, int64_t in_pitch,
+                   int64_t out_pitch, void* hip_stream) {
+  if (!plan || !ins || !outs) return fail(MPX_ERR_ARG, "plan/ins/outs is NULL");
+  if (n < 1 || n > MPX_MAX_FRAMES || n > plan->max_batch)
+    return fail(MPX_ERR_SHAPE, "n = " + std::to_string(n) + " outside 1.." +
+                                   std::to_string(std::min(MPX_MAX_FRAMES, plan->max_batch)));
+  const mpx_variant_info& inf = plan->info;
+  if (in_pitch < plan->in_w || out_pitch < inf.out_w) return fail(MPX_ERR_ARG, "pitch smaller than the row width");
+  for (int i = 0; i < n; ++i)
+    if (!ins[i] || !outs[i]) return fail(MPX_ERR_ARG, "frame " + std::to_string(i) + " has a NULL plane");
+  HIP_TRY(hipSetDevice(plan->ctx->device));
+  mpx::Io d{};
+  d.ntab = n;
+  for (int i = 0; i < n; ++i) {
+    d.tab_in[i] = ins[i];
+    d.tab_out[i] = outs[i];
+  }
+  d.in = ins[0];
+  d.out = outs[0];
+  d.in_pitch = in_pitch;
+  d.in_cstride = in_pitch * plan->in_h;
+  d.out_pitch = out_pitch;
+  d.out_cstride = out_pitch * inf.out_h;
+  d.in_dt = plan->in_type;
+  d.out_dt = plan->out_type;
+  d.in_max = plan->in_max;
+  d.W = plan->in_w;
+  d.H = plan->in_h;
+  d.batch = n;
+  return dispatch(plan, d, static_cast<hipStream_t>(hip_stream));
+}
+
+static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st) {
+  const mpx_variant_info& inf = plan->info;
   hipError_t e = hipSuccess;
   if (!inf.applies) {
     e = mpx::launch_copy(d, inf.planes, st);

@@ -12,10 +12,17 @@ namespace mpx {
 
 enum DType : int { DT_U8 = 0, DT_U16 = 1, DT_F16 = 2, DT_F32 = 3 };
 
-// Device-side description of one launch's planes (strides in elements).
+constexpr int kMaxTab = 16;  // frames per launch through a pointer table
+
+// Device-side description of one launch's planes (strides in elements).  Frames are
+// either `bstride` apart from `in` / `out`, or (ntab > 0) at the addresses of the
+// pointer tables, which travel by value in the kernel arguments.
 struct Io {
   const void* in;
   void* out;
+  const void* tab_in[kMaxTab];
+  void* tab_out[kMaxTab];
+  int ntab;
   int64_t in_pitch, in_cstride, in_bstride;
   int64_t out_pitch, out_cstride, out_bstride;
   int32_t* buckets;   // optional, [batch][sites] rows
@@ -25,6 +32,17 @@ struct Io {
   int W, H;           // input plane size
   int batch;
 };
+
+__device__ __host__ __forceinline__ int elem_bytes(int dt) { return dt == DT_F32 ? 4 : (dt == DT_U8 ? 1 : 2); }
+
+__device__ __forceinline__ const void* frame_in(const Io& io, int f) {
+  if (io.ntab) return io.tab_in[f];
+  return static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * elem_bytes(io.in_dt);
+}
+__device__ __forceinline__ void* frame_out(const Io& io, int f) {
+  if (io.ntab) return io.tab_out[f];
+  return static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * elem_bytes(io.out_dt);
+}
 
 __device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, float in_max) {
   switch (dt) {

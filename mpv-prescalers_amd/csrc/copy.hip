@@ -11,10 +11,8 @@ __global__ __launch_bounds__(256) void copy_kernel(Io io, int planes) {
   const int fc = blockIdx.z;  // frame * planes + plane
   if (x >= io.W) return;
   const int frame = fc / planes, c = fc - frame * planes;
-  const int esz_in = io.in_dt == DT_F32 ? 4 : (io.in_dt == DT_U8 ? 1 : 2);
-  const int esz_out = io.out_dt == DT_F32 ? 4 : 2;
-  const void* in = static_cast<const char*>(io.in) + (frame * io.in_bstride + c * io.in_cstride) * esz_in;
-  void* out = static_cast<char*>(io.out) + (frame * io.out_bstride + c * io.out_cstride) * esz_out;
+  const void* in = static_cast<const char*>(frame_in(io, frame)) + c * io.in_cstride * elem_bytes(io.in_dt);
+  void* out = static_cast<char*>(frame_out(io, frame)) + c * io.out_cstride * elem_bytes(io.out_dt);
   const float v = load_elem(in, static_cast<int64_t>(y) * io.in_pitch + x, io.in_dt, io.in_max);
   store_elem(out, static_cast<int64_t>(y) * io.out_pitch + x, io.out_dt, v);
 }

@@ -38,6 +38,9 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 struct NPass {
   const void* src;
   void* dst;
+  const void* tab_src[kMaxTab];  // per-frame bases when ntab_src > 0
+  void* tab_dst[kMaxTab];        // per-frame bases when ntab_dst > 0
+  int ntab_src, ntab_dst;
   int64_t src_pitch, src_bstride, dst_pitch, dst_bstride;
   int src_dt, dst_dt;
   float src_max;
@@ -110,11 +113,12 @@ __device__ __forceinline__ void store_result(const NPass& p, void* dst, int x, i
 }
 
 __device__ __forceinline__ const void* frame_src(const NPass& p, int f) {
-  const int esz = p.src_dt == DT_F32 ? 4 : (p.src_dt == DT_U8 ? 1 : 2);
-  return static_cast<const char*>(p.src) + static_cast<int64_t>(f) * p.src_bstride * esz;
+  if (p.ntab_src) return p.tab_src[f];
+  return static_cast<const char*>(p.src) + static_cast<int64_t>(f) * p.src_bstride * elem_bytes(p.src_dt);
 }
 __device__ __forceinline__ void* frame_dst(const NPass& p, int f) {
-  return static_cast<char*>(p.dst) + static_cast<int64_t>(f) * p.dst_bstride * (p.dst_dt == DT_F32 ? 4 : 2);
+  if (p.ntab_dst) return p.tab_dst[f];
+  return static_cast<char*>(p.dst) + static_cast<int64_t>(f) * p.dst_bstride * elem_bytes(p.dst_dt);
 }
 
 // ----------------------------------------------------------------------------- FP32 VALU
@@ -212,7 +216,7 @@ __device__ __forceinline__ float4v contract(const WFrag<KC, IS_BF16>& w, const t
 // The window operand is x' * XSCALE and the weights carry WSCALE (host side), so the
 // accumulators hold ACC = XSCALE*WSCALE*(s + b): powers of two that keep the FP16 low
 // parts out of the subnormal range (MFMA flushes FP16 subnormal inputs).
-template <int WH, int AXIS, bool IS_BF16, int TERMS, int NSL, int TPW>
+template <int WH, int AXIS, bool IS_BF16, int TERMS, int NSL, int TPW, bool SCALED>
 __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __restrict__ w_hi,
                                                       const void* __restrict__ w_lo,
                                                       const float* __restrict__ bias, float xscale, float inv_acc,
@@ -266,7 +270,8 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
       b2[t][i] = bias[2 * nn + 1];
     }
   }
-  const float acc_one = 1.0f / inv_acc;  // the accumulator's scale: 1 + |t| -> acc_one + |acc|
+  // accumulator scale: exp2 argument acc*inv_acc, and 1 + |t| -> acc_one + |acc|
+  const float acc_one = SCALED ? 1.0f / inv_acc : 1.0f;
 
   const int per_frame = tiles_x * tiles_y;
   for (int tileid = blockIdx.x; tileid < total_tiles; tileid += gridDim.x) {
@@ -293,7 +298,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
       T* rl = xlo + (TERMS >= 3 ? threadIdx.x * XS : 0);
 #pragma unroll
       for (int k = 0; k < KP; ++k) {
-        const float v = k < K ? xv[k] * xscale : 0.f;
+        const float v = k < K ? (SCALED ? xv[k] * xscale : xv[k]) : 0.f;
         const T h = O::cvt(v);
         rh[k] = h;
         if (TERMS >= 3) rl[k] = O::cvt(v - O::back(h));
@@ -317,7 +322,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
         const float4v a2 = contract<KC, IS_BF16, TERMS>(w2[t], bh, bl, b2[t]);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const float e = __builtin_amdgcn_exp2f(a1[i] * inv_acc);
+          const float e = __builtin_amdgcn_exp2f(SCALED ? a1[i] * inv_acc : a1[i]);
           // t/(1+|t|) with t = a2/acc_one  ==  a2/(acc_one + |a2|)
           const float q = a2[i] * __builtin_amdgcn_rcpf(acc_one + fabsf(a2[i]));
           ws += e;
@@ -372,26 +377,36 @@ static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t s
     return hipGetLastError();
   }
   const int total = tiles_x * tiles_y * p.batch;
-  const int grid = min(total, cu_count() * 2);
   const int nt = nw.nns / 16;
-#define MPX_NN(BF, TERMS, NSL, TPW)                                                                    \
-  hipLaunchKernelGGL((nnedi3_mfma<WH, AXIS, BF, TERMS, NSL, TPW>), dim3(grid), dim3(256), 0, st, p, nw.w, \
-                     nw.w_lo, nw.bias, nw.xscale, 1.0f / (nw.xscale * nw.wscale), tiles_x, tiles_y, total)
-#define MPX_NN_NT(BF, TERMS)                     \
-  switch (nt) {                                  \
-    case 1: MPX_NN(BF, TERMS, 1, 1); break;      \
-    case 2: MPX_NN(BF, TERMS, 2, 1); break;      \
-    case 4: MPX_NN(BF, TERMS, 4, 1); break;      \
-    case 8: MPX_NN(BF, TERMS, 4, 2); break;      \
-    case 16: MPX_NN(BF, TERMS, 4, 4); break;     \
-    default: return hipErrorInvalidValue;        \
+  const float inv_acc = 1.0f / (nw.xscale * nw.wscale);
+  // Neuron tiles are split over NSL waves only when one wave cannot hold them all
+  // (nns >= 128); otherwise every wave owns all neurons for its own pixel groups.
+  // Persistent grid: as many blocks as the occupancy of the instance allows per CU.
+#define MPX_NN(BF, TERMS, NSL, TPW, SC)                                                               \
+  {                                                                                                   \
+    auto kern = nnedi3_mfma<WH, AXIS, BF, TERMS, NSL, TPW, SC>;                                       \
+    int per_cu = 0;                                                                                   \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) \
+      per_cu = 1;                                                                                     \
+    const int grid = min(total, cu_count() * per_cu);                                                 \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.bias, nw.xscale,       \
+                       inv_acc, tiles_x, tiles_y, total);                                             \
+  }
+#define MPX_NN_NT(BF, TERMS, SC)                    \
+  switch (nt) {                                     \
+    case 1: MPX_NN(BF, TERMS, 1, 1, SC) break;      \
+    case 2: MPX_NN(BF, TERMS, 1, 2, SC) break;      \
+    case 4: MPX_NN(BF, TERMS, 1, 4, SC) break;      \
+    case 8: MPX_NN(BF, TERMS, 2, 4, SC) break;      \
+    case 16: MPX_NN(BF, TERMS, 4, 4, SC) break;     \
+    default: return hipErrorInvalidValue;           \
   }
   if (nw.precision == 1) {
-    MPX_NN_NT(false, 1)
+    MPX_NN_NT(false, 1, false)
   } else if (nw.precision == 4) {
-    MPX_NN_NT(false, 3)
+    MPX_NN_NT(false, 3, true)
   } else if (nw.precision == 2) {
-    MPX_NN_NT(true, 3)
+    MPX_NN_NT(true, 3, false)
   } else {
     return hipErrorInvalidValue;
   }
@@ -413,6 +428,14 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
     p.W = io.W;
     p.H = io.H;
     p.batch = io.batch;
+    if (io.ntab) {
+      p.ntab_src = io.ntab;
+      for (int i = 0; i < io.ntab; ++i) p.tab_src[i] = io.tab_in[i];
+      if (!dx) {
+        p.ntab_dst = io.ntab;
+        for (int i = 0; i < io.ntab; ++i) p.tab_dst[i] = io.tab_out[i];
+      }
+    }
     if (dx) {
       p.dst = tmp;
       p.dst_pitch = io.W;
@@ -443,6 +466,14 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
       p.src_dt = io.in_dt;
       p.src_max = io.in_max;
       p.H = io.H;
+      if (io.ntab) {
+        p.ntab_src = io.ntab;
+        for (int i = 0; i < io.ntab; ++i) p.tab_src[i] = io.tab_in[i];
+      }
+    }
+    if (io.ntab) {
+      p.ntab_dst = io.ntab;
+      for (int i = 0; i < io.ntab; ++i) p.tab_dst[i] = io.tab_out[i];
     }
     p.W = io.W;
     p.batch = io.batch;

@@ -39,11 +39,12 @@ int nnedi3_prepare(mpx_ctx* ctx, int nns, int wh, int precision, NnediWeights* o
   out->nns = nns;
   out->wh = wh;
   out->precision = precision;
-  // FP16 forms carry power-of-two operand scales (window x256, weights x256) so that the
-  // low parts of the split stay normal FP16 numbers; BF16 has FP32's exponent range.
+  // The FP16 split form carries power-of-two operand scales (window x256, weights x256)
+  // so that the low parts stay normal FP16 numbers (MFMA flushes FP16 subnormal inputs);
+  // BF16 has FP32's exponent range, the single-product FP16 form has no low parts.
   const bool f16 = precision == MPX_PREC_FP16 || precision == MPX_PREC_FP16X3;
-  out->xscale = f16 ? 256.f : 1.f;
-  out->wscale = f16 ? 256.f : 1.f;
+  out->xscale = precision == MPX_PREC_FP16X3 ? 256.f : 1.f;
+  out->wscale = precision == MPX_PREC_FP16X3 ? 256.f : 1.f;
   const double ws = out->wscale, accs = static_cast<double>(out->xscale) * out->wscale;
   auto W1 = [&](int n, int k) { return static_cast<double>(t[n * (2 * K + 2) + k]) * kLog2e; };
   auto W2 = [&](int n, int k) { return static_cast<double>(t[n * (2 * K + 2) + K + k]); };

@@ -20,7 +20,6 @@ namespace mpx {
 constexpr int RT_X = 64;
 constexpr int RT_Y = 4;
 
-__device__ __forceinline__ int esz(int dt) { return dt == DT_F32 ? 4 : (dt == DT_U8 ? 1 : 2); }
 
 // Stage P planes of a frame into LDS: tile[c][r][col] = plane c at
 // (clamp(ox + col), clamp(oy + r)), PH x PW per plane.
@@ -65,7 +64,7 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   __shared__ float tile[P * PN];
   const int f = blockIdx.z;
   const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
-  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz(io.in_dt);
+  const void* in = frame_in(io, f);
   stage_planes<P>(tile, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - HALO, y0 - HALO);
   __syncthreads();
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
@@ -105,8 +104,8 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
       }
     }
   }
-  const int esz_out = io.out_dt == DT_F32 ? 4 : 2;
-  char* ob = static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * esz_out;
+  const int esz_out = elem_bytes(io.out_dt);
+  char* ob = static_cast<char*>(frame_out(io, f));
 #pragma unroll
   for (int c = 0; c < P; ++c) {
     void* o = ob + c * io.out_cstride * esz_out;
@@ -139,12 +138,16 @@ hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, 
 
 // res = sum_{p < N^2/2} (s_p + s_{N^2-1-p}) * w_p, w_p = texel p/4, component p%4,
 // clamped to [0,1] (ravu-r3.hook:120-146).  s(c, i) = plane c sample i.
+template <int N> struct RavuLut {
+  static constexpr int NTEX = (N * N / 2 + 3) / 4;  // texels per LUT row: 2 / 5 / 8 for r2 / r3 / r4
+};
+
 template <int N, int P, typename S>
 __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
 #pragma unroll
   for (int c = 0; c < P; ++c) res[c] = 0.f;
 #pragma unroll
-  for (int t = 0; t < N * N / 8; ++t) {
+  for (int t = 0; t < RavuLut<N>::NTEX; ++t) {
     const float4 w = load_lut4(lrow, t);
     const float a[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   __shared__ float tile[P * PN];
   const int f = blockIdx.z;
   const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
-  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz(io.in_dt);
+  const void* in = frame_in(io, f);
   stage_planes<P>(tile, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - (R - 1),
                   y0 - (R - 1));
   __syncthreads();
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   const int row = bucket_row<RULE_LOG2>(bk, 648);
   if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * io.W + x] = row;
   float res[P];
-  ravu_taps<N, P>(lut + static_cast<int64_t>(row) * (N * N / 8) * 4,
+  ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4,
                   [&](int c, int i) { return tile[c * PN + off(i)]; }, res);
   const int64_t plane = static_cast<int64_t>(io.W) * io.H;
   float* o = int11 + f * int11_frame + static_cast<int64_t>(y) * io.W + x;
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(256) void ravu_step23_kernel(Io io, const __half* _
   __shared__ float i11[P * PN];  // ravu_int11
   const int f = blockIdx.z;
   const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
-  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz(io.in_dt);
+  const void* in = frame_in(io, f);
   stage_planes<P>(hk, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - R, y0 - R);
   const int64_t plane = static_cast<int64_t>(io.W) * io.H;
   stage_planes<P>(i11, PH, PW, int11 + f * int11_frame, DT_F32, 1.f, io.W, plane, io.W, io.H, x0 - R, y0 - R);
@@ -230,7 +233,7 @@ __global__ __launch_bounds__(256) void ravu_step23_kernel(Io io, const __half* _
                                                     [&](int i) { return luma_d<CH>(src(i), 0, PN); });
     const int row = bucket_row<RULE_LOG2>(bk, 648);
     if (io.buckets) io.buckets[f * io.bucket_bstride + plane + static_cast<int64_t>(y) * io.W + x] = row;
-    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * (N * N / 8) * 4, [&](int c, int i) { return src(i)[c * PN]; },
+    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
                     res10);
   }
   {
@@ -244,12 +247,12 @@ __global__ __launch_bounds__(256) void ravu_step23_kernel(Io io, const __half* _
                                                     [&](int i) { return luma_d<CH>(src(i), 0, PN); });
     const int row = bucket_row<RULE_LOG2>(bk, 648);
     if (io.buckets) io.buckets[f * io.bucket_bstride + 2 * plane + static_cast<int64_t>(y) * io.W + x] = row;
-    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * (N * N / 8) * 4, [&](int c, int i) { return src(i)[c * PN]; },
+    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
                     res01);
   }
   // combine (ravu-r3.hook:426-437): (2x,2y) input, (2x,2y+1) int01, (2x+1,2y) int10, (2x+1,2y+1) int11
-  const int esz_out = io.out_dt == DT_F32 ? 4 : 2;
-  char* ob = static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * esz_out;
+  const int esz_out = elem_bytes(io.out_dt);
+  char* ob = static_cast<char*>(frame_out(io, f));
 #pragma unroll
   for (int c = 0; c < P; ++c) {
     void* o = ob + c * io.out_cstride * esz_out;

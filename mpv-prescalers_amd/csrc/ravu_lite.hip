@@ -31,9 +31,7 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   const int frame = blockIdx.z;
   const int x0 = blockIdx.x * LITE_TX;
   const int y0 = blockIdx.y * LITE_TY;
-  const char* inb = static_cast<const char*>(io.in);
-  const int esz_in = io.in_dt == DT_F32 ? 4 : (io.in_dt == DT_U8 ? 1 : 2);
-  const void* in = inb + static_cast<int64_t>(frame) * io.in_bstride * esz_in;
+  const void* in = frame_in(io, frame);
 
   for (int i = threadIdx.x; i < PH * PW; i += blockDim.x) {
     const int r = i / PW, c = i - r * PW;
@@ -123,8 +121,7 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
     for (int k = 0; k < 4; ++k) r4[k] = fminf(fmaxf(r4[k], 0.f), 1.f);
   }
   // res[2*dx + dy] -> output (2x+dx, 2y+dy)
-  char* outb = static_cast<char*>(io.out);
-  void* out = outb + static_cast<int64_t>(frame) * io.out_bstride * (io.out_dt == DT_F32 ? 4 : 2);
+  void* out = frame_out(io, frame);
   const int64_t o0 = static_cast<int64_t>(2 * y) * io.out_pitch + 2 * x;
   store_pair(out, o0, io.out_dt, r4[0], r4[2]);
   store_pair(out, o0 + io.out_pitch, io.out_dt, r4[1], r4[3]);

@@ -43,14 +43,16 @@ __device__ __forceinline__ Phase phase_of(int o, int in, int out) {
   return p;
 }
 
-// FILTER LINEAR fetch of group g at texel coordinates (9g + tx, ty), tx in [0, 8],
-// ty unclamped; clamp-to-edge on the texture.
-__device__ __forceinline__ float4 lut_bilinear(const __half* lut, int width, int rows, int g, float tx, float ty) {
+// FILTER LINEAR fetch of group g at texel coordinates (9g + tx, 9*row + ty), tx, ty in
+// [0, 8]; the integer row base is kept apart so the phase fraction keeps full FP32
+// precision.  Rows clamp to the texture (row can be -1 or past the last bucket).
+__device__ __forceinline__ float4 lut_bilinear(const __half* lut, int width, int rows, int g, float tx, int row,
+                                               float ty) {
   const float fx0 = floorf(tx), fy0 = floorf(ty);
   const float ax = tx - fx0, ay = ty - fy0;
   const int x0 = 9 * g + static_cast<int>(fx0);
   const int x1 = min(x0 + 1, width - 1);
-  const int y0i = static_cast<int>(fy0);
+  const int y0i = 9 * row + static_cast<int>(fy0);
   const int y0 = min(max(y0i, 0), rows - 1), y1 = min(max(y0i + 1, 0), rows - 1);
   const float4 v00 = load_lut4(lut, static_cast<int64_t>(y0) * width + x0);
   const float4 v01 = load_lut4(lut, static_cast<int64_t>(y0) * width + x1);
@@ -96,8 +98,7 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   const int ox0 = blockIdx.x * ZT_X, oy0 = blockIdx.y * ZT_Y;
   const int bx0 = phase_of(ox0, io.W, out_w).base;
   const int by0 = phase_of(oy0, io.H, out_h).base;
-  const int esz_in = io.in_dt == DT_F32 ? 4 : (io.in_dt == DT_U8 ? 1 : 2);
-  const void* in = static_cast<const char*>(io.in) + static_cast<int64_t>(f) * io.in_bstride * esz_in;
+  const void* in = frame_in(io, f);
   for (int i = threadIdx.x; i < P * PN; i += blockDim.x) {
     const int c = i / PN, j = i - c * PN;
     const int r = j / PW, col = j - r * PW;
@@ -125,19 +126,19 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(oy) * out_w + ox] = row;
   auto s = [&](int c, int i) { return tile[c * PN + (lby + i % N) * PW + lbx + i / N]; };
   const float txf = 8.f * phx.f, txi = 8.f * (1.f - phx.f);
-  const float tyf = 9.f * row + 8.f * phy.f, tyi = 9.f * row + 8.f * (1.f - phy.f);
+  const float tyf = 8.f * phy.f, tyi = 8.f * (1.f - phy.f);
   float res[P], hi[P], lo[P], hi2[P], lo2[P];
 #pragma unroll
   for (int c = 0; c < P; ++c) res[c] = hi[c] = lo[c] = hi2[c] = lo2[c] = 0.f;
 #pragma unroll
   for (int g = 0; g < GROUPS; ++g) {
-    const float4 wf4 = lut_bilinear(lut, WIDTH, ROWS, g, txf, tyf);
-    const float4 wi4 = lut_bilinear(lut, WIDTH, ROWS, g, txi, tyi);
+    const float4 wf4 = lut_bilinear(lut, WIDTH, ROWS, g, txf, row, tyf);
+    const float4 wi4 = lut_bilinear(lut, WIDTH, ROWS, g, txi, row, tyi);
     const float wf[4] = {wf4.x, wf4.y, wf4.z, wf4.w}, wi[4] = {wi4.x, wi4.y, wi4.z, wi4.w};
     float af[4] = {0.f, 0.f, 0.f, 0.f}, ai[4] = {0.f, 0.f, 0.f, 0.f};
     if (AR) {
-      const float4 a4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txf, tyf);
-      const float4 b4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txi, tyi);
+      const float4 a4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txf, row, tyf);
+      const float4 b4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txi, row, tyi);
       af[0] = a4.x; af[1] = a4.y; af[2] = a4.z; af[3] = a4.w;
       ai[0] = b4.x; ai[1] = b4.y; ai[2] = b4.z; ai[3] = b4.w;
     }
@@ -165,8 +166,8 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
       }
     }
   }
-  const int esz_out = io.out_dt == DT_F32 ? 4 : 2;
-  char* ob = static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * esz_out;
+  const int esz_out = elem_bytes(io.out_dt);
+  char* ob = static_cast<char*>(frame_out(io, f));
 #pragma unroll
   for (int c = 0; c < P; ++c) {
     float v = res[c];

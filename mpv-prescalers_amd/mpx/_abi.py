@@ -23,7 +23,8 @@ CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 # every exported symbol declared in include/mpx.h
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
-           "mpx_run_ex", "mpx_last_error")
+           "mpx_run_ex", "mpx_run_frames", "mpx_last_error")
+MAX_FRAMES = 16
 
 
 class VariantInfo(ctypes.Structure):
@@ -73,6 +74,7 @@ def _load():
         "mpx_plan_destroy": (None, [vp]),
         "mpx_run": (i32, [vp, vp, i64, i64, vp, i64, i64, i32, vp]),
         "mpx_run_ex": (i32, [vp, ctypes.POINTER(MpxIo), vp]),
+        "mpx_run_frames": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), i64, i64, vp]),
         "mpx_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

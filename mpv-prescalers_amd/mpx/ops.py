@@ -119,6 +119,30 @@ class Plan:
         return (out, bk) if buckets else out
 
 
+    def run_frames(self, xs, outs=None, stream=None):
+        """One launch (per pass) over separate same-shape frames (up to 16): each x in
+        ``xs`` is a device tensor of the plan's input shape, results go to ``outs``
+        (allocated if None).  No copies: the frame addresses travel in the kernel args."""
+        n = len(xs)
+        if n < 1 or n > min(_abi.MAX_FRAMES, self.max_batch):
+            raise ValueError("run_frames takes 1..%d frames" % min(_abi.MAX_FRAMES, self.max_batch))
+        if outs is None:
+            outs = [torch.empty(self.out_shape(x), dtype=self.out_dtype, device=x.device) for x in xs]
+        pitch_in = xs[0].stride(-2)
+        pitch_out = outs[0].stride(-2)
+        for x, o in zip(xs, outs):
+            if x.dtype != self.in_dtype or tuple(x.shape[-2:]) != (self.in_h, self.in_w) or not x.is_contiguous():
+                raise ValueError("frame does not match the plan")
+            if x.stride(-2) != pitch_in or o.stride(-2) != pitch_out or not o.is_contiguous():
+                raise ValueError("frames must share row pitches")
+        ins = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+        ots = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        if stream is None:
+            stream = torch.cuda.current_stream(xs[0].device)
+        check(LIB.mpx_run_frames(self._h, n, ins, ots, pitch_in, pitch_out, ctypes.c_void_p(stream.cuda_stream)))
+        return outs
+
+
 class Context:
     """Device binding + resident weights + HBM budget (bytes, 0 = unlimited)."""
 

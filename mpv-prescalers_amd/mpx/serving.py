@@ -1,0 +1,219 @@
+"""Serving loop: continuous batching of upscale requests on one GPU.
+
+Requests (a device frame + variant name + display size) arrive at any time.  A scheduler
+thread keeps the weights resident (one Context), admits queued requests in arrival order
+while the bytes of in-flight outputs stay within the HBM budget, groups the admitted ones
+by plan (variant, size, dtypes, precision), and launches each group -- up to 16 frames at
+independent addresses per launch (``mpx_run_frames``) -- on the least recently used of
+several HIP streams, so launches of different variants and sizes overlap on the device
+while new arrivals keep being admitted.  A completion thread waits on each launch's event
+in order and stamps every request's finish time; latency = finish - arrival.
+
+Each (plan key, stream) pair owns its plan, since a plan's scratch serves one stream at a
+time.  All streams use the default priority.
+"""
+import collections
+import threading
+import time
+
+import torch
+
+from .ops import Context, default_target
+
+
+class Request:
+    __slots__ = ("name", "frame", "target", "precision", "bits", "out_dtype", "t_arrival", "t_admit", "t_done",
+                 "out", "bytes", "key", "error", "src_stream", "_done")
+
+    def __init__(self, name, frame, target, precision, bits, out_dtype):
+        self.name = name
+        self.frame = frame
+        self.target = target
+        self.precision = precision
+        self.bits = bits
+        self.out_dtype = out_dtype
+        self.t_arrival = time.perf_counter()
+        self.t_admit = self.t_done = None
+        self.out = None
+        self.bytes = 0
+        self.key = None
+        self.error = None
+        self.src_stream = torch.cuda.current_stream(frame.device)  # where the frame was produced
+        self._done = threading.Event()
+
+    def wait(self, timeout=None):
+        if not self._done.wait(timeout):
+            raise TimeoutError("request not finished")
+        if self.error is not None:
+            raise self.error
+        return self.out
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_arrival
+
+
+class Server:
+    """Continuous-batching request loop over one device.
+
+    max_batch: frames per launch (<= 16); streams: concurrent HIP streams;
+    hbm_budget: bytes for weights + plan scratch (the Context's budget) and, separately,
+    ``admit_bytes`` for the outputs of admitted, unfinished requests."""
+
+    def __init__(self, device=0, hbm_budget=16 << 30, admit_bytes=None, streams=4, max_batch=8,
+                 plan_batch=None):
+        self.device = torch.device("cuda", device) if isinstance(device, int) else device
+        self.ctx = Context(self.device.index or 0, hbm_budget=hbm_budget)
+        self.streams = [torch.cuda.Stream(self.device) for _ in range(streams)]
+        self.max_batch = max(1, min(16, max_batch))
+        self.plan_batch = plan_batch or self.max_batch
+        self.admit_bytes = admit_bytes if admit_bytes is not None else hbm_budget // 2
+        self._plans = {}
+        self._queue = collections.deque()
+        self._inflight = 0
+        self._cv = threading.Condition()
+        self._done_q = collections.deque()
+        self._done_cv = threading.Condition()
+        self._next_stream = 0
+        self._stop = False
+        self._paused = False
+        self.launches = 0
+        self._sched = threading.Thread(target=self._schedule, daemon=True)
+        self._compl = threading.Thread(target=self._complete, daemon=True)
+        self._sched.start()
+        self._compl.start()
+
+    # ------------------------------------------------------------------ client side
+    def submit(self, name, frame, target=None, precision="fp32", bits=10, out_dtype=None):
+        """Queue one frame ((H, W) luma or (3, H, W)); returns a Request (``wait()``)."""
+        in_h, in_w = frame.shape[-2:]
+        tgt = tuple(target) if target is not None else default_target(name, in_w, in_h)
+        if out_dtype is None:
+            out_dtype = frame.dtype if frame.dtype in (torch.float16, torch.float32) else torch.float32
+        r = Request(name, frame, tgt, precision, bits, out_dtype)
+        with self._cv:
+            self._queue.append(r)
+            self._cv.notify()
+        return r
+
+    def pause(self):
+        """Stop admitting (queued requests wait); for building a burst deterministically."""
+        with self._cv:
+            self._paused = True
+
+    def resume(self):
+        with self._cv:
+            self._paused = False
+            self._cv.notify()
+
+    def close(self):
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        self._sched.join()
+        with self._done_cv:
+            self._done_cv.notify_all()
+        self._compl.join()
+        for p in self._plans.values():
+            p.close()
+        self._plans.clear()
+        self.ctx.close()
+
+    # ------------------------------------------------------------------ scheduler
+    def _plan(self, key, si):
+        p = self._plans.get((key, si))
+        if p is None:
+            name, in_w, in_h, tgt, in_dt, out_dt, prec, bits = key
+            p = self.ctx.plan(name, in_w, in_h, tgt, in_dt, out_dt, prec, self.plan_batch, bits)
+            self._plans[(key, si)] = p
+        return p
+
+    def _schedule(self):
+        while True:
+            with self._cv:
+                while (not self._queue or self._paused) and not self._stop:
+                    self._cv.wait()
+                if self._stop and not self._queue:
+                    return
+                admitted = []
+                # admit in arrival order while the output bytes fit the budget
+                while self._queue:
+                    r = self._queue[0]
+                    if r.key is None:
+                        in_h, in_w = r.frame.shape[-2:]
+                        key = (r.name, in_w, in_h, r.target, r.frame.dtype, r.out_dtype, r.precision, r.bits)
+                        try:
+                            info = self._plan(key, 0).info
+                        except Exception as e:  # unknown / unavailable variant, bad size, budget
+                            self._queue.popleft()
+                            r.error = e
+                            r.t_done = time.perf_counter()
+                            r._done.set()
+                            continue
+                        r.key = key
+                        r.bytes = info["planes"] * info["out_w"] * info["out_h"] * torch.finfo(r.out_dtype).bits // 8
+                    if self._inflight + r.bytes > self.admit_bytes and self._inflight > 0:
+                        break
+                    self._queue.popleft()
+                    self._inflight += r.bytes
+                    r.t_admit = time.perf_counter()
+                    admitted.append(r)
+            groups = collections.OrderedDict()
+            for r in admitted:
+                groups.setdefault(r.key, []).append(r)
+            for key, reqs in groups.items():
+                for i in range(0, len(reqs), self.max_batch):
+                    self._launch(key, reqs[i:i + self.max_batch])
+
+    def _launch(self, key, reqs):
+        si = self._next_stream
+        self._next_stream = (si + 1) % len(self.streams)
+        st = self.streams[si]
+        try:
+            plan = self._plan(key, si)
+            for src in {r.src_stream for r in reqs}:
+                st.wait_stream(src)  # the frames were produced on the submitters' streams
+            with torch.cuda.stream(st):
+                outs = [torch.empty(plan.out_shape(r.frame), dtype=r.out_dtype, device=self.device) for r in reqs]
+                plan.run_frames([r.frame for r in reqs], outs, stream=st)
+                ev = torch.cuda.Event()
+                ev.record(st)
+            for r, o in zip(reqs, outs):
+                r.out = o
+            self.launches += 1
+            with self._done_cv:
+                self._done_q.append((ev, reqs))
+                self._done_cv.notify()
+        except Exception as e:  # a failing launch fails its requests, the loop goes on
+            for r in reqs:
+                r.error = e
+                r.t_done = time.perf_counter()
+                self._release(r)
+                r._done.set()
+
+    def _release(self, r):
+        with self._cv:
+            self._inflight -= r.bytes
+            self._cv.notify()
+
+    def _complete(self):
+        while True:
+            with self._done_cv:
+                while not self._done_q:
+                    if self._stop and not self._sched.is_alive():
+                        return
+                    self._done_cv.wait(0.05)
+                ev, reqs = self._done_q.popleft()
+            ev.synchronize()
+            t = time.perf_counter()
+            for r in reqs:
+                r.t_done = t
+                self._release(r)
+                r._done.set()
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))]

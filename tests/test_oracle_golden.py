@@ -140,3 +140,19 @@ def test_nnedi3_transpose_symmetry():
     from oracle.nnedi3 import double_x, double_y
     img = make_plane(20, 14, 9)
     assert np.array_equal(double_x(img, 32, 6), double_y(img.T.copy(), 32, 6).T)
+
+
+def test_c_restatement_agrees_with_oracle():
+    """The C FP32 restatement (the CPU baseline) against the FP64 oracle: NNEDI3 within the
+    FP32 cancellation of the shader's E[x^2]-m^2 variance; RAVU-lite identical except on
+    the few sites whose bucket FP32 decides differently."""
+    from mpx.frames import make_plane
+    from oracle import cbuild
+    from oracle.nnedi3 import nnedi3
+    from oracle.ravu import ravu_lite
+    img = make_plane(96, 64, 3).astype(np.float32)
+    assert np.abs(cbuild.nnedi3(img, 32, 6) - nnedi3(img.astype(np.float64), 32, 6)).max() < 2e-3
+    for ar in (False, True):
+        c = cbuild.ravu_lite(img, 3, ar)
+        o, _ = ravu_lite(img.astype(np.float64), 3, ar=ar)
+        assert (np.abs(c - o) > 1e-5).mean() < 1e-3
