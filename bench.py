@@ -5,9 +5,10 @@ one 1920x1080 FP16 plane -> 3840x2160, FP16-MFMA form (FP32 accumulation); a ste
 one frame through both passes.  Reported: output Mpixel/s, the plain FP32 VALU form's
 Mpixel/s and the ratio, p50/p99 per-frame latency, the dominant kernel's roofline
 (double_x pass: 2*K*2*nns FLOP per predicted pixel, W x 2H predicted pixels per
-launch), secondary variants, and the FP64 oracle's CPU throughput on the same workload.
+launch), secondary variants, the serving loop, and the C FP32 restatement's CPU throughput on
+the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel-only]
 """
 import argparse
 import json
@@ -172,6 +173,54 @@ def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
             "speedup_vs_one_frame_per_launch_async": round(sat / single_async, 2), "launches": launches}
 
 
+def _double_x(ctx, torch):
+    """The double_x pass alone: FP32 W x 2H intermediate in (as inside the two-pass
+    pipeline), 2W x 2H FP16 out."""
+    dev = torch.device("cuda:0")
+    xi = torch.rand((2 * H, W), dtype=torch.float32, device=dev)
+    xo = torch.empty((2 * H, 2 * W), dtype=torch.float16, device=dev)
+    plan = ctx.plan(HEADLINE, W, 2 * H, target=(2 * W, 2 * H), in_dtype=torch.float32,
+                    out_dtype=torch.float16, precision="fp16")
+    assert plan.info["scale_x"] == 2 and plan.info["scale_y"] == 1
+    return lambda: plan.run(xi, out=xo)
+
+
+def dominant_kernel(ctx, steps, torch):
+    """Roofline of the dominant kernel, timed with events on the stream it runs on.
+    Algorithmic work: 2*K*2*nns FLOP per predicted pixel x W*2H predicted pixels.
+    HBM traffic per launch comes from the hardware-counter summary committed beside the
+    kernel-trace profiles (FETCH_SIZE + WRITE_SIZE of this launch, calibrated on a plain
+    copy), when present."""
+    ktimes, _ = timed(_double_x(ctx, torch), steps, 5, torch)
+    kms = sum(ktimes) / len(ktimes)
+    flop = nnedi3_flop_per_px(HEADLINE) * W * 2 * H
+    achieved = flop / (kms * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_nnedi3_double_x_nns32_1920x2160.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    return {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP16, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP16, 4), "traffic": traffic,
+            "kernel": "nnedi3_mfma double_x (fp16, nns32, win8x4)", "kernel_ms": round(kms, 5),
+            "flop_per_launch": flop, "algorithmic_io_bytes": W * 2 * H * 4 + 2 * W * 2 * H * 2}
+
+
+def kernel_only(ctx, steps, torch):
+    """The dominant kernel `steps` times, then a plain device copy of 256 MiB `steps`
+    times (the calibration for the FETCH_SIZE / WRITE_SIZE counters)."""
+    fn = _double_x(ctx, torch)
+    for _ in range(steps):
+        fn()
+    src = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel_launches": steps, "copy_bytes_each_way": src.numel() * 4}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +229,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the dominant kernel (double_x pass) and a plain copy of known size, for "
+                         "hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (frames shard trivially across "
@@ -199,6 +251,9 @@ def main():
     out = torch.empty((2 * H, 2 * W), dtype=torch.float16, device=dev)
     steps, warm = args.steps, args.warmup
 
+    if args.kernel_only:
+        return kernel_only(ctx, args.steps, torch)
+
     plans = {p: ctx.plan(HEADLINE, W, H, in_dtype=torch.float16, out_dtype=torch.float16, precision=p)
              for p in ("fp16", "fp32")}
     res = {}
@@ -207,21 +262,7 @@ def main():
         times, wall = timed(lambda: plan.run(x, out=out), n, warm if p == "fp16" else 3, torch)
         res[p] = {"ms_per_step": wall / n * 1e3, "times": times, "mpx_s": 4 * W * H * n / wall / 1e6}
 
-    # dominant kernel: the double_x pass alone (FP32 W x 2H intermediate in, as inside the
-    # two-pass pipeline), timed with events on the launch stream
-    xi = torch.rand((2 * H, W), dtype=torch.float32, device=dev)
-    xo = torch.empty((2 * H, 2 * W), dtype=torch.float16, device=dev)
-    px_plan = ctx.plan(HEADLINE, W, 2 * H, target=(2 * W, 2 * H), in_dtype=torch.float32,
-                       out_dtype=torch.float16, precision="fp16")
-    assert px_plan.info["scale_x"] == 2 and px_plan.info["scale_y"] == 1
-    ktimes, _ = timed(lambda: px_plan.run(xi, out=xo), max(20, steps // 2), 5, torch)
-    kms = sum(ktimes) / len(ktimes)
-    flop = nnedi3_flop_per_px(HEADLINE) * W * 2 * H
-    achieved = flop / (kms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP16, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP16, 4), "traffic": None,
-                "kernel": "nnedi3_mfma double_x (fp16, nns32, win8x4)", "kernel_ms": round(kms, 5),
-                "flop_per_launch": flop}
+    roofline = dominant_kernel(ctx, max(20, steps // 2), torch)
 
     secondary = {}
     if not args.no_secondary:

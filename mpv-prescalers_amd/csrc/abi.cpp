@@ -187,9 +187,17 @@ int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int targ
   *out = nullptr;
   const Variant* v = find_variant(name);
   if (!v) return fail(MPX_ERR_UNKNOWN_VARIANT, std::string("unknown variant '") + (name ? name : "(null)") + "'");
-  if (!v->available)
-    return fail(MPX_ERR_UNAVAILABLE, std::string(name) +
-                                         ": weights are not shipped by the reference (listed in .MISSING_LARGE_BLOBS)");
+  if (!v->available) {
+    // The reference does not ship these weights; a user-supplied table in the weights
+    // directory (same MPXW format, (2592, 45, 4)) makes the variant usable.
+    FILE* fchk = std::fopen((ctx->wdir + "/ravu_zoom_lut" + std::to_string(v->radius) + "_ar.bin").c_str(), "rb");
+    if (!fchk)
+      return fail(MPX_ERR_UNAVAILABLE, std::string(name) +
+                                           ": weights are not shipped by the reference (listed in "
+                                           ".MISSING_LARGE_BLOBS); supply ravu_zoom_lut" +
+                                           std::to_string(v->radius) + "_ar.bin in the weights directory");
+    std::fclose(fchk);
+  }
   if (in_w <= 0 || in_h <= 0 || target_w <= 0 || target_h <= 0 || in_w > 32768 || in_h > 32768 ||
       target_w > 65536 || target_h > 65536)
     return fail(MPX_ERR_SHAPE, "sizes out of range");

@@ -53,6 +53,57 @@ __device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, f
   }
 }
 
+// Halo tile of P planes, PH x PW samples each, taken at (ox + col, oy + r) clamped to the
+// W x H plane (clamp-to-edge sampling) and staged to LDS as tile[c][r][col], by NT
+// threads.  `issue` starts every load of the calling thread with no branch on the
+// element type: each sample is fetched as the aligned dword that holds it (which never
+// leaves the sample's own page), its bit offset kept beside it.  `commit` extracts and
+// converts the samples and writes LDS.  A tile therefore costs one memory latency, and a
+// persistent block can issue its next tile while it computes this one.
+template <int P, int PH, int PW, int NT>
+struct HaloTile {
+  static constexpr int PN = PH * PW;
+  static constexpr int N = P * PN;
+  static constexpr int E = (N + NT - 1) / NT;
+  uint32_t v[E];
+  uint32_t sh[E];
+
+  __device__ __forceinline__ void issue(const void* base, int dt, int64_t pitch, int64_t cstride, int W, int H,
+                                        int ox, int oy) {
+    const int lsh = dt == DT_F32 ? 2 : (dt == DT_U8 ? 0 : 1);  // log2 element bytes
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = min(static_cast<int>(threadIdx.x) + e * NT, N - 1);  // tail lanes repeat the last sample
+      const int c = P == 1 ? 0 : i / PN;
+      const int j = i - c * PN;
+      const int r = j / PW, col = j - r * PW;
+      const int gx = min(max(ox + col, 0), W - 1);
+      const int gy = min(max(oy + r, 0), H - 1);
+      const uintptr_t a = b + (static_cast<uintptr_t>(c * cstride + static_cast<int64_t>(gy) * pitch + gx) << lsh);
+      v[e] = *reinterpret_cast<const uint32_t*>(a & ~static_cast<uintptr_t>(3));
+      sh[e] = static_cast<uint32_t>(a & 3) * 8;
+    }
+  }
+  __device__ __forceinline__ void commit(float* tile, int dt, float in_max) const {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = static_cast<int>(threadIdx.x) + e * NT;
+      if (E * NT == N || i < N) {
+        const uint32_t w = v[e] >> sh[e];
+        float f;
+        switch (dt) {
+          case DT_F32: f = __uint_as_float(v[e]); break;
+          case DT_F16: f = __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu))); break;
+          case DT_U8: f = static_cast<float>(w & 0xffu) / in_max; break;  // value / full scale
+          default: f = static_cast<float>(w & 0xffffu) / in_max; break;
+        }
+        tile[i] = f;
+      }
+    }
+  }
+};
+
 __device__ __forceinline__ void store_elem(void* p, int64_t idx, int dt, float v) {
   if (dt == DT_F32)
     static_cast<float*>(p)[idx] = v;

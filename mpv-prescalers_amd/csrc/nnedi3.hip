@@ -70,15 +70,7 @@ struct Geo {
 };
 
 template <int WH, int AXIS>
-__device__ __forceinline__ void load_tile(float* tile, const NPass& p, const void* src, int x0, int y0) {
-  using G = Geo<WH, AXIS>;
-  for (int i = threadIdx.x; i < G::TH * G::TW; i += blockDim.x) {
-    const int r = i / G::TW, c = i - r * G::TW;
-    const int gx = min(max(x0 + G::OX + c, 0), p.W - 1);
-    const int gy = min(max(y0 + G::OY + r, 0), p.H - 1);
-    tile[i] = load_elem(src, static_cast<int64_t>(gy) * p.src_pitch + gx, p.src_dt, p.src_max);
-  }
-}
+using NTile = HaloTile<1, Geo<WH, AXIS>::TH, Geo<WH, AXIS>::TW, 256>;
 
 // mean, 1/std (0 below the variance threshold) and std*... of a K-sample window held in
 // registers; the window is replaced by its centred, normalised form.
@@ -130,7 +122,11 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
   __shared__ float tile[G::TH * G::TW];
   const int f = blockIdx.z;
   const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY;
-  load_tile<WH, AXIS>(tile, p, frame_src(p, f), x0, y0);
+  {
+    NTile<WH, AXIS> ht;
+    ht.issue(frame_src(p, f), p.src_dt, p.src_pitch, 0, p.W, p.H, x0 + G::OX, y0 + G::OY);
+    ht.commit(tile, p.src_dt, p.src_max);
+  }
   __syncthreads();
   const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;
   const int x = x0 + tx, y = y0 + ty;
@@ -273,14 +269,23 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   // accumulator scale: exp2 argument acc*inv_acc, and 1 + |t| -> acc_one + |acc|
   const float acc_one = SCALED ? 1.0f / inv_acc : 1.0f;
 
+  // The halo of the block's next tile is loaded into registers while this one computes.
   const int per_frame = tiles_x * tiles_y;
+  NTile<WH, AXIS> ht;
+  auto issue = [&](int id) {
+    const int ff = id / per_frame, rr = id - ff * per_frame;
+    ht.issue(frame_src(p, ff), p.src_dt, p.src_pitch, 0, p.W, p.H, (rr % tiles_x) * NX + G::OX,
+             (rr / tiles_x) * NY + G::OY);
+  };
+  if (static_cast<int>(blockIdx.x) < total_tiles) issue(blockIdx.x);
   for (int tileid = blockIdx.x; tileid < total_tiles; tileid += gridDim.x) {
     const int f = tileid / per_frame;
     const int rem = tileid - f * per_frame;
     const int y0 = (rem / tiles_x) * NY, x0 = (rem % tiles_x) * NX;
     __syncthreads();  // previous iteration done with the LDS
-    load_tile<WH, AXIS>(tile, p, frame_src(p, f), x0, y0);
+    ht.commit(tile, p.src_dt, p.src_max);
     __syncthreads();
+    if (tileid + static_cast<int>(gridDim.x) < total_tiles) issue(tileid + gridDim.x);
 
     // one thread per pixel: normalised window -> 16-bit rows (hi, lo), zero padded to KP
     const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;

@@ -97,7 +97,8 @@ int nnedi3_prepare(mpx_ctx* ctx, int nns, int wh, int precision, NnediWeights* o
             lo[i] = bf16_bits(static_cast<float>(v - static_cast<double>(bf16_float(h))));
           }
         }
-    const std::string tag = f16 ? "/fp16" : "/bf16";
+    // the key carries the weight scale: FP16 and FP16X3 pack the same table differently
+    const std::string tag = (f16 ? "/fp16_x" : "/bf16_x") + std::to_string(static_cast<long long>(ws));
     rc = ctx->upload(name + tag + "_hi", hi.data(), nel * 2, &p);
     if (rc) return rc;
     out->w = p;

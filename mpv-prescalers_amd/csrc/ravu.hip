@@ -4,8 +4,9 @@
 // (centre = input pixel), 24x3x3 buckets, two LUT texels per tap.
 // RAVU (ravu-r3.hook:15-437): kernel 1 predicts int11 (x+1/2, y+1/2) from the 2r x 2r
 // window into an FP32 scratch plane; kernel 2 predicts int10 (x+1/2, y) and int01
-// (x, y+1/2) on the 45-degree lattice mixing the input and int11, and writes the
-// interleaved 2x output (the combine pass, :426-437).  Both read LDS tiles staged with
+// (x, y+1/2) on the 45-degree lattice mixing the input and int11, and write the
+// interleaved 2x output (the combine pass, :426-437), one kernel per prediction (each
+// writes one of the two output rows of a pixel).  All read LDS tiles staged with
 // clamp-to-edge halos, each texture clamped on its own size as mpv's sampler does.
 //
 // Channel forms (ravu-r3-yuv.hook, ravu-r3-rgb.hook): the hash sees plane 0 (yuv) or
@@ -22,18 +23,13 @@ constexpr int RT_Y = 4;
 
 
 // Stage P planes of a frame into LDS: tile[c][r][col] = plane c at
-// (clamp(ox + col), clamp(oy + r)), PH x PW per plane.
-template <int P>
-__device__ __forceinline__ void stage_planes(float* tile, int PH, int PW, const void* base, int dt, float in_max,
-                                             int64_t pitch, int64_t cstride, int W, int H, int ox, int oy) {
-  const int n = PH * PW;
-  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
-    const int c = i / n, j = i - c * n;
-    const int r = j / PW, col = j - r * PW;
-    const int gx = min(max(ox + col, 0), W - 1);
-    const int gy = min(max(oy + r, 0), H - 1);
-    tile[i] = load_elem(base, c * cstride + static_cast<int64_t>(gy) * pitch + gx, dt, in_max);
-  }
+// (clamp(ox + col), clamp(oy + r)), PH x PW per plane; all loads in flight at once.
+template <int P, int PH, int PW>
+__device__ __forceinline__ void stage_planes(float* tile, const void* base, int dt, float in_max, int64_t pitch,
+                                             int64_t cstride, int W, int H, int ox, int oy) {
+  HaloTile<P, PH, PW, 256> h;
+  h.issue(base, dt, pitch, cstride, W, H, ox, oy);
+  h.commit(tile, dt, in_max);
 }
 
 // Hash-plane value from P staged planes at LDS offset o (plane stride n).
@@ -65,7 +61,7 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   const int f = blockIdx.z;
   const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
   const void* in = frame_in(io, f);
-  stage_planes<P>(tile, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - HALO, y0 - HALO);
+  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - HALO, y0 - HALO);
   __syncthreads();
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
@@ -174,7 +170,7 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   const int f = blockIdx.z;
   const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
   const void* in = frame_in(io, f);
-  stage_planes<P>(tile, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - (R - 1),
+  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - (R - 1),
                   y0 - (R - 1));
   __syncthreads();
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
@@ -202,9 +198,13 @@ __device__ __host__ constexpr int lat2x(int X, int Y) { return STEP == 2 ? X + Y
 template <int N, int STEP>
 __device__ __host__ constexpr int lat2y(int X, int Y) { return STEP == 2 ? Y - X : Y - X + 1; }
 
-template <int R, int CH>
-__global__ __launch_bounds__(256) void ravu_step23_kernel(Io io, const __half* __restrict__ lut,
-                                                          const float* __restrict__ int11, int64_t int11_frame) {
+// step 2 (int10, STEP = 2) or step 3 (int01, STEP = 3) on the lattice, and the combine
+// for one output row: STEP 2 writes row 2y = (input, int10) pairs, STEP 3 writes row
+// 2y+1 = (int01, int11) pairs (ravu-r3.hook:426-437), so both kernels store contiguous
+// pairs and each holds one prediction's registers only.
+template <int R, int CH, int STEP>
+__global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* __restrict__ lut,
+                                                           const float* __restrict__ int11, int64_t int11_frame) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
   constexpr int PW = RT_X + 2 * R + 1, PH = RT_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
@@ -213,52 +213,40 @@ __global__ __launch_bounds__(256) void ravu_step23_kernel(Io io, const __half* _
   const int f = blockIdx.z;
   const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
   const void* in = frame_in(io, f);
-  stage_planes<P>(hk, PH, PW, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - R, y0 - R);
   const int64_t plane = static_cast<int64_t>(io.W) * io.H;
-  stage_planes<P>(i11, PH, PW, int11 + f * int11_frame, DT_F32, 1.f, io.W, plane, io.W, io.H, x0 - R, y0 - R);
+  {
+    HaloTile<P, PH, PW, 256> a, b;  // both tiles' loads in flight together
+    a.issue(in, io.in_dt, io.in_pitch, io.in_cstride, io.W, io.H, x0 - R, y0 - R);
+    b.issue(int11 + f * int11_frame, DT_F32, io.W, plane, io.W, io.H, x0 - R, y0 - R);
+    a.commit(hk, io.in_dt, io.in_max);
+    b.commit(i11, DT_F32, 1.f);
+  }
   __syncthreads();
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= io.W || y >= io.H) return;
   const int base = (ty + R) * PW + tx + R;  // LDS offset of (x, y)
-  float res10[P], res01[P];
-  {
-    // step 2: int10
-    auto src = [&](int i) -> const float* {
-      const int X = i / N, Y = i % N;
-      const int px = lat2x<N, 2>(X, Y), py = lat2y<N, 2>(X, Y);
-      return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
-    };
-    const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
-                                                    [&](int i) { return luma_d<CH>(src(i), 0, PN); });
-    const int row = bucket_row<RULE_LOG2>(bk, 648);
-    if (io.buckets) io.buckets[f * io.bucket_bstride + plane + static_cast<int64_t>(y) * io.W + x] = row;
-    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
-                    res10);
-  }
-  {
-    // step 3: int01
-    auto src = [&](int i) -> const float* {
-      const int X = i / N, Y = i % N;
-      const int px = lat2x<N, 3>(X, Y), py = lat2y<N, 3>(X, Y);
-      return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
-    };
-    const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
-                                                    [&](int i) { return luma_d<CH>(src(i), 0, PN); });
-    const int row = bucket_row<RULE_LOG2>(bk, 648);
-    if (io.buckets) io.buckets[f * io.bucket_bstride + 2 * plane + static_cast<int64_t>(y) * io.W + x] = row;
-    ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
-                    res01);
-  }
-  // combine (ravu-r3.hook:426-437): (2x,2y) input, (2x,2y+1) int01, (2x+1,2y) int10, (2x+1,2y+1) int11
+  auto src = [&](int i) -> const float* {
+    const int X = i / N, Y = i % N;
+    const int px = lat2x<N, STEP>(X, Y), py = lat2y<N, STEP>(X, Y);
+    return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
+  };
+  const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
+                                                  [&](int i) { return luma_d<CH>(src(i), 0, PN); });
+  const int row = bucket_row<RULE_LOG2>(bk, 648);
+  if (io.buckets) io.buckets[f * io.bucket_bstride + (STEP - 1) * plane + static_cast<int64_t>(y) * io.W + x] = row;
+  float res[P];
+  ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
+                  res);
   const int esz_out = elem_bytes(io.out_dt);
   char* ob = static_cast<char*>(frame_out(io, f));
 #pragma unroll
   for (int c = 0; c < P; ++c) {
     void* o = ob + c * io.out_cstride * esz_out;
-    const int64_t o0 = static_cast<int64_t>(2 * y) * io.out_pitch + 2 * x;
-    store_pair(o, o0, io.out_dt, hk[c * PN + base], res10[c]);
-    store_pair(o, o0 + io.out_pitch, io.out_dt, res01[c], i11[c * PN + base]);
+    if (STEP == 2)
+      store_pair(o, static_cast<int64_t>(2 * y) * io.out_pitch + 2 * x, io.out_dt, hk[c * PN + base], res[c]);
+    else
+      store_pair(o, static_cast<int64_t>(2 * y + 1) * io.out_pitch + 2 * x, io.out_dt, res[c], i11[c * PN + base]);
   }
 }
 
@@ -270,7 +258,10 @@ hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, flo
     hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);       \
     hipError_t e = hipGetLastError();                                                                          \
     if (e != hipSuccess) return e;                                                                             \
-    hipLaunchKernelGGL((ravu_step23_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);      \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);  \
+    e = hipGetLastError();                                                                                     \
+    if (e != hipSuccess) return e;                                                                             \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);  \
     return hipGetLastError();                                                                                  \
   }
   MPX_RAVU(2, 0) MPX_RAVU(2, 1) MPX_RAVU(2, 2) MPX_RAVU(3, 0) MPX_RAVU(3, 1) MPX_RAVU(3, 2) MPX_RAVU(4, 0)

@@ -33,11 +33,10 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   const int y0 = blockIdx.y * LITE_TY;
   const void* in = frame_in(io, frame);
 
-  for (int i = threadIdx.x; i < PH * PW; i += blockDim.x) {
-    const int r = i / PW, c = i - r * PW;
-    const int gx = min(max(x0 + c - HALO, 0), io.W - 1);
-    const int gy = min(max(y0 + r - HALO, 0), io.H - 1);
-    tile[i] = load_elem(in, static_cast<int64_t>(gy) * io.in_pitch + gx, io.in_dt, io.in_max);
+  {
+    HaloTile<1, PH, PW, 256> h;
+    h.issue(in, io.in_dt, io.in_pitch, 0, io.W, io.H, x0 - HALO, y0 - HALO);
+    h.commit(tile, io.in_dt, io.in_max);
   }
   __syncthreads();
 

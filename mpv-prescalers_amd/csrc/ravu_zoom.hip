@@ -99,12 +99,10 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   const int bx0 = phase_of(ox0, io.W, out_w).base;
   const int by0 = phase_of(oy0, io.H, out_h).base;
   const void* in = frame_in(io, f);
-  for (int i = threadIdx.x; i < P * PN; i += blockDim.x) {
-    const int c = i / PN, j = i - c * PN;
-    const int r = j / PW, col = j - r * PW;
-    const int gx = min(max(bx0 - (R - 1) + col, 0), io.W - 1);
-    const int gy = min(max(by0 - (R - 1) + r, 0), io.H - 1);
-    tile[i] = load_elem(in, c * io.in_cstride + static_cast<int64_t>(gy) * io.in_pitch + gx, io.in_dt, io.in_max);
+  {
+    HaloTile<P, PH, PW, 256> h;
+    h.issue(in, io.in_dt, io.in_pitch, io.in_cstride, io.W, io.H, bx0 - (R - 1), by0 - (R - 1));
+    h.commit(tile, io.in_dt, io.in_max);
   }
   __syncthreads();
   // hash every base pixel of the tile (unclamped LUT row: bucket 9-row blocks are clamped

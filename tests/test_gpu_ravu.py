@@ -129,3 +129,29 @@ def test_ravu_zoom_ar_r3_unavailable():
     with pytest.raises(MpxError) as e:
         mpx.upscale("ravu-zoom-ar-r3", x, target=(32, 32))
     assert e.value.code == -3
+
+
+def test_ravu_zoom_ar_r3_with_supplied_table(tmp_path):
+    """ravu-zoom-ar-r3's AR table is not shipped; with a user-supplied table in the weights
+    directory the variant runs.  A synthetic non-negative table (|main r3 table|) checks
+    the r3 AR path against the oracle using the same table."""
+    import os
+    import mpx
+    from oracle.common import WEIGHTS_DIR, load_weights
+    from oracle.ravu import ravu_zoom
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("extract", os.path.join(WEIGHTS_DIR, "extract.py"))
+    ext = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ext)
+    for f in os.listdir(WEIGHTS_DIR):
+        if f.endswith(".bin"):
+            os.symlink(os.path.join(WEIGHTS_DIR, f), str(tmp_path / f))
+    table = np.abs(load_weights("ravu_zoom_lut3")).astype(np.float32)
+    ext.write_bin(str(tmp_path / "ravu_zoom_lut3_ar.bin"), table)
+    ctx = mpx.Context(0, weights_dir=str(tmp_path))
+    img = _frame(80, 48, seed=9, planes=1)
+    plan = ctx.plan("ravu-zoom-ar-r3", 80, 48, target=(120, 72))
+    out = plan.run(torch.from_numpy(img).cuda()).cpu().numpy()
+    ref, _ = ravu_zoom(img.astype(np.float64), 3, 120, 72, ar=True,
+                       ar_lut=table.astype(np.float16).astype(np.float64))
+    assert np.abs(out - ref).max() <= 5e-5
