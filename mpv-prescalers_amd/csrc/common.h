@@ -44,6 +44,28 @@ __device__ __forceinline__ void* frame_out(const Io& io, int f) {
   return static_cast<char*>(io.out) + static_cast<int64_t>(f) * io.out_bstride * elem_bytes(io.out_dt);
 }
 
+// Each of the 8 XCDs (own L2 each) takes thread blocks round-robin in launch order.
+// xcd_tile maps launch-order block L of T to a tile so that every XCD walks one contiguous
+// range of tiles: neighbouring tiles, which share halo rows, then meet in the same L2.
+// A bijection on [0, T) for any T.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int L, int T) {
+  const int q = T / kXcds, r = T % kXcds;
+  const int x = L % kXcds, j = L / kXcds;
+  return x * q + min(x, r) + j;
+}
+
+// Tile (x, y) and frame z of this block in a 1-D launch of nx * ny * nz blocks.
+struct BlockXYZ {
+  int x, y, z;
+};
+__device__ __forceinline__ BlockXYZ block_xyz(int nx, int ny, int nz) {
+  const int per = nx * ny;
+  const int t = xcd_tile(static_cast<int>(blockIdx.x), per * nz);
+  const int z = t / per, r = t - z * per;
+  return {r % nx, r / nx, z};
+}
+
 __device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, float in_max) {
   switch (dt) {
     case DT_F32: return static_cast<const float*>(p)[idx];

@@ -15,14 +15,16 @@ namespace mpx {
 //   FP16/BF16: w  = 16-bit [nns][2][K] high parts of wscale*(w1*log2e, w2),
 //              w_lo = the same layout holding the residuals (w - high), for the
 //              three-product split; bias = float [nns][2] xscale*wscale*(b1*log2e, b2)
-//   FP8:       w  = e4m3 [nns][2][K] of w / scale; scale = float [nns][2]; bias as above
+//   FP8:       w  = OCP e4m3 [nns][2][128 slots]: slots [0,K) and [K,2K) both hold
+//              w * 2^-e (e per neuron and matrix, so max|w| * 2^-e is in (224, 448]),
+//              the rest zero; wexp = int [nns][2] E8M0 scales 127 + e; bias as above
 struct NnediWeights {
   int nns = 0, wh = 0, precision = 0;
   float xscale = 1.f, wscale = 1.f;  // power-of-two operand scales of the MFMA forms
   const void* w = nullptr;
   const void* w_lo = nullptr;
   const float* bias = nullptr;
-  const float* scale = nullptr;
+  const int* wexp = nullptr;
 };
 
 int nnedi3_prepare(mpx_ctx* ctx, int nns, int wh, int precision, NnediWeights* out);

@@ -120,8 +120,9 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
   using G = Geo<WH, AXIS>;
   constexpr int K = G::K;
   __shared__ float tile[G::TH * G::TW];
-  const int f = blockIdx.z;
-  const int x0 = blockIdx.x * NX, y0 = blockIdx.y * NY;
+  const BlockXYZ bk3 = block_xyz((p.W + NX - 1) / NX, (p.H + NY - 1) / NY, p.batch);
+  const int f = bk3.z;
+  const int x0 = bk3.x * NX, y0 = bk3.y * NY;
   {
     NTile<WH, AXIS> ht;
     ht.issue(frame_src(p, f), p.src_dt, p.src_pitch, 0, p.W, p.H, x0 + G::OX, y0 + G::OY);
@@ -158,16 +159,18 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
 
 // ----------------------------------------------------------------------------- MFMA
 
-// 16-bit operand types: F16 (IS_BF16 = false) or BF16.
-template <bool IS_BF16> struct Op;
-template <> struct Op<false> {
+// Operand forms of the contraction.
+enum Form : int { F_F16 = 0, F_BF16 = 1, F_FP8 = 2 };
+
+template <int FORM> struct Op;
+template <> struct Op<F_F16> {
   typedef _Float16 T;
   typedef half8v V8;
   __device__ static float4v mma(V8 a, V8 b, float4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
   __device__ static T cvt(float v) { return static_cast<_Float16>(v); }
   __device__ static float back(T v) { return static_cast<float>(v); }
 };
-template <> struct Op<true> {
+template <> struct Op<F_BF16> {
   typedef __bf16 T;
   typedef bf8v V8;
   __device__ static float4v mma(V8 a, V8 b, float4v c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
@@ -175,89 +178,187 @@ template <> struct Op<true> {
   __device__ static float back(T v) { return static_cast<float>(v); }
 };
 
-// K in chunks of 32 (v_mfma_f32_16x16x32): K = 32 -> 1 chunk, K = 48 -> 2 chunks with the
-// second half of the last chunk zero (weights and window both zero there).
-template <int K> struct KChunks {
+// The contraction of one 16-neuron tile with one 16-pixel group, per operand form.
+//
+// F16 / BF16 (v_mfma_f32_16x16x32_{f16,bf16}): K in chunks of 32, zero padded (K = 48 ->
+// two chunks); lane l holds A row (neuron) l%16 and B column (pixel) l%16 at
+// k = 32*c + 8*(l/16) + 0..7.  TERMS = 3 adds high*low + low*high products of the split
+// operands.  The window rows sit in LDS as 16-bit values, XS elements apart.
+template <int FORM, int K, int TERMS>
+struct Mma {
+  using O = Op<FORM>;
+  using T = typename O::T;
   static constexpr int KC = (K + 31) / 32;
   static constexpr int KP = 32 * KC;
-  static constexpr int XS = KP + 8;  // LDS row stride of the window matrix (padding vs bank conflicts)
-};
-
-// Operand fragments of one neuron tile (16 neurons) for one matrix (w1 or w2): lane l
-// holds row (neuron) l%16, k = 32*c + 8*(l/16) + 0..7, high and low parts.
-template <int KC, bool IS_BF16>
-struct WFrag {
-  typename Op<IS_BF16>::V8 hi[KC], lo[KC];
-};
-
-template <int KC, bool IS_BF16, int TERMS>
-__device__ __forceinline__ float4v contract(const WFrag<KC, IS_BF16>& w, const typename Op<IS_BF16>::V8* bh,
-                                            const typename Op<IS_BF16>::V8* bl, float4v acc) {
-  using O = Op<IS_BF16>;
-#pragma unroll
-  for (int c = 0; c < KC; ++c) {
-    acc = O::mma(w.hi[c], bh[c], acc);
-    if (TERMS >= 3) {
-      acc = O::mma(w.hi[c], bl[c], acc);
-      acc = O::mma(w.lo[c], bh[c], acc);
-    }
-  }
-  return acc;
-}
-
-// Block: 4 waves.  The NT = nns/16 neuron tiles are split into NSL slices (NSL = min(NT,4));
-// waves with the same neuron slice share the 16 pixel groups of a tile, 4/NSL pixel
-// slices.  TPW = NT/NSL neuron tiles per wave, held in VGPRs for the whole launch.
-// TERMS = 1: one product of 16-bit operands; TERMS = 3: high*high + high*low + low*high.
-// The window operand is x' * XSCALE and the weights carry WSCALE (host side), so the
-// accumulators hold ACC = XSCALE*WSCALE*(s + b): powers of two that keep the FP16 low
-// parts out of the subnormal range (MFMA flushes FP16 subnormal inputs).
-template <int WH, int AXIS, bool IS_BF16, int TERMS, int NSL, int TPW, bool SCALED>
-__global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __restrict__ w_hi,
-                                                      const void* __restrict__ w_lo,
-                                                      const float* __restrict__ bias, float xscale, float inv_acc,
-                                                      int tiles_x, int tiles_y, int total_tiles) {
-  using G = Geo<WH, AXIS>;
-  using O = Op<IS_BF16>;
-  using T = typename O::T;
-  constexpr int K = G::K;
-  constexpr int KC = KChunks<K>::KC;
-  constexpr int KP = KChunks<K>::KP;
-  constexpr int XS = KChunks<K>::XS;
-  constexpr int NPX = NX * NY;            // 256 pixels per tile
-  constexpr int NGROUP = NPX / 16;        // 16 pixel groups
-  constexpr int PSL = 4 / NSL;            // pixel slices
-  __shared__ float tile[G::TH * G::TW];
-  __shared__ __attribute__((aligned(16))) T xhi[NPX * XS];
-  __shared__ __attribute__((aligned(16))) T xlo[TERMS >= 3 ? NPX * XS : 8];
-  __shared__ float part[NSL][NPX][2];
-
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nslice = wave % NSL;
-  const int pslice = wave / NSL;
-  const int row = lane & 15, quad = lane >> 4;
-  const T zero = static_cast<T>(0.0f);
-
-  // this wave's weight fragments and the accumulator initial values (scaled biases)
-  WFrag<KC, IS_BF16> w1[TPW], w2[TPW];
-  float4v b1[TPW], b2[TPW];
-  const T* Wh = static_cast<const T*>(w_hi);
-  const T* Wl = static_cast<const T*>(w_lo);
-#pragma unroll
-  for (int t = 0; t < TPW; ++t) {
-    const int n = (nslice * TPW + t) * 16 + row;          // neuron of this lane's A row
+  static constexpr int XS = KP + 8;  // LDS row stride (elements; padding vs bank conflicts)
+  static constexpr int LO_ROWS = TERMS >= 3;
+  struct W {
+    typename O::V8 hi[KC], lo[KC];
+  };
+  struct B {
+    typename O::V8 h[KC], l[KC];
+  };
+  __device__ static void load_w(W& w, const void* wh, const void* wl, const int* /*sc*/, int n, int mtx, int quad) {
+    const T* H = static_cast<const T*>(wh);
+    const T* L = static_cast<const T*>(wl);
+    const T zero = static_cast<T>(0.0f);
 #pragma unroll
     for (int c = 0; c < KC; ++c)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int k = 32 * c + 8 * quad + j;
         const bool in = k < K;
-        w1[t].hi[c][j] = in ? Wh[(static_cast<int64_t>(n) * 2 + 0) * K + k] : zero;
-        w2[t].hi[c][j] = in ? Wh[(static_cast<int64_t>(n) * 2 + 1) * K + k] : zero;
-        w1[t].lo[c][j] = (TERMS >= 3 && in) ? Wl[(static_cast<int64_t>(n) * 2 + 0) * K + k] : zero;
-        w2[t].lo[c][j] = (TERMS >= 3 && in) ? Wl[(static_cast<int64_t>(n) * 2 + 1) * K + k] : zero;
+        w.hi[c][j] = in ? H[(static_cast<int64_t>(n) * 2 + mtx) * K + k] : zero;
+        w.lo[c][j] = (TERMS >= 3 && in) ? L[(static_cast<int64_t>(n) * 2 + mtx) * K + k] : zero;
       }
+  }
+  // one pixel's normalised window (scaled by xscale) -> its LDS row(s)
+  __device__ static void write_row(T* rh, T* rl, const float* xv, float xscale, bool scaled) {
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const float v = k < K ? (scaled ? xv[k] * xscale : xv[k]) : 0.f;
+      const T h = O::cvt(v);
+      rh[k] = h;
+      if (TERMS >= 3) rl[k] = O::cvt(v - O::back(h));
+    }
+  }
+  __device__ static void load_b(B& b, const T* xh, const T* xl, int px, int quad) {
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      b.h[c] = *reinterpret_cast<const typename O::V8*>(xh + px * XS + 32 * c + 8 * quad);
+      if (TERMS >= 3) b.l[c] = *reinterpret_cast<const typename O::V8*>(xl + px * XS + 32 * c + 8 * quad);
+    }
+  }
+  __device__ static float4v contract(const W& w, const B& b, float4v acc) {
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      acc = O::mma(w.hi[c], b.h[c], acc);
+      if (TERMS >= 3) {
+        acc = O::mma(w.hi[c], b.l[c], acc);
+        acc = O::mma(w.lo[c], b.h[c], acc);
+      }
+    }
+    return acc;
+  }
+};
+
+// FP8 (v_mfma_scale_f32_16x16x128_f8f6f4, OCP e4m3 on both sides, one instruction per
+// tile): the 128 K slots hold [window high parts (K) | window low parts (K) | zeros]
+// against [weights (K) | the same weights (K) | zeros], so one product carries the split
+// window.  Lane l holds A row l%16 / B column l%16 in slots 32*(l/16) + 0..31 (the same
+// slots on both sides, so the slot order inside the instruction does not matter).
+// Weights are stored as e4m3 of w * 2^-e_n with a power-of-two scale per neuron and
+// matrix, applied by the instruction's per-lane E8M0 scale (127 + e_n); the window is
+// stored as e4m3 of x' * 2^5 and scaled back by 2^-5 the same way.
+constexpr int kFp8XExp = 5;
+typedef int int8v __attribute__((ext_vector_type(8)));
+
+template <int K, int TERMS>
+struct Mma<F_FP8, K, TERMS> {
+  typedef uint8_t T;
+  static constexpr int XS = 128 + 16;  // bytes per LDS row
+  static constexpr int LO_ROWS = 0;
+  struct W {
+    int8v a;
+    int sc;
+  };
+  struct B {
+    int8v b;
+  };
+  __device__ static void load_w(W& w, const void* wh, const void* /*wl*/, const int* sc, int n, int mtx, int quad) {
+    const int4* src = reinterpret_cast<const int4*>(static_cast<const uint8_t*>(wh) +
+                                                    (static_cast<int64_t>(n) * 2 + mtx) * 128 + 32 * quad);
+    const int4 lo = src[0], hi = src[1];
+    w.a = int8v{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    w.sc = sc[n * 2 + mtx];
+  }
+  __device__ static void write_row(T* rh, T* /*rl*/, const float* xv, float /*xscale*/, bool /*scaled*/) {
+    constexpr float XSC = static_cast<float>(1 << kFp8XExp);
+    uint32_t hv[(K + 3) / 4], lv[(K + 3) / 4];
+#pragma unroll
+    for (int k = 0; k < K; k += 2) {
+      const float v0 = xv[k] * XSC, v1 = xv[k + 1] * XSC;
+      const uint32_t h = __builtin_amdgcn_cvt_pk_fp8_f32(v0, v1, 0, false);
+      const float r0 = v0 - __builtin_amdgcn_cvt_f32_fp8(static_cast<int>(h), 0);
+      const float r1 = v1 - __builtin_amdgcn_cvt_f32_fp8(static_cast<int>(h), 1);
+      const uint32_t l = __builtin_amdgcn_cvt_pk_fp8_f32(r0, r1, 0, false);
+      if (k % 4 == 0) {
+        hv[k / 4] = h & 0xffffu;
+        lv[k / 4] = l & 0xffffu;
+      } else {
+        hv[k / 4] |= h << 16;
+        lv[k / 4] |= l << 16;
+      }
+    }
+    // slots [0, K) high parts, [K, 2K) low parts, [2K, 128) zero; K is a multiple of 16
+    uint32_t* r = reinterpret_cast<uint32_t*>(rh);
+#pragma unroll
+    for (int i = 0; i < 32; ++i) r[i] = i < K / 4 ? hv[i] : (i < K / 2 ? lv[i - K / 4] : 0u);
+  }
+  __device__ static void load_b(B& b, const T* xh, const T* /*xl*/, int px, int quad) {
+    const int4* src = reinterpret_cast<const int4*>(xh + px * XS + 32 * quad);
+    const int4 lo = src[0], hi = src[1];
+    b.b = int8v{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  }
+  __device__ static float4v contract(const W& w, const B& b, float4v acc) {
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(w.a, b.b, acc, 0, 0, 0, w.sc, 0, 127 - kFp8XExp);
+  }
+};
+
+// Orders this wave's LDS writes before its later LDS reads (a wave's LDS operations run in
+// issue order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Block: 4 waves, tile NX x NY = 64 x 4 pixels, wave w owning pixel row w (pixel groups
+// 4w..4w+3 of 16).  The NT = nns/16 neuron tiles are split into NSL slices; TPW = NT/NSL
+// neuron tiles per wave, held in VGPRs for the whole launch.
+//  * NSL = 1 (nns <= 64): every wave runs all neurons for its own 64 pixels, so the
+//    normalised rows it writes to LDS are read back by the same wave and the softmax sums
+//    stay in registers: one block barrier per tile (after the halo tile lands).
+//  * NSL > 1: waves with the same neuron slice share the 16 pixel groups (4/NSL pixel
+//    slices); partial sums meet in LDS.
+// TERMS = 1: one product of 16-bit operands; TERMS = 3: high*high + high*low + low*high.
+// The window operand is x' * XSCALE and the weights carry WSCALE (host side), so the
+// accumulators hold ACC = XSCALE*WSCALE*(s + b): powers of two that keep the FP16 low
+// parts out of the subnormal range (MFMA flushes FP16 subnormal inputs).
+// The halo tile is double-buffered: the block's next tile is fetched into registers while
+// this one computes and lands in the other buffer.
+template <int WH, int AXIS, int FORM, int TERMS, int NSL, int TPW, bool SCALED>
+__global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __restrict__ w_hi,
+                                                      const void* __restrict__ w_lo, const int* __restrict__ wsc,
+                                                      const float* __restrict__ bias, float xscale, float inv_acc,
+                                                      int tiles_x, int tiles_y, int total_tiles) {
+  using G = Geo<WH, AXIS>;
+  constexpr int K = G::K;
+  using M = Mma<FORM, K, TERMS>;
+  using T = typename M::T;
+  constexpr int XS = M::XS;
+  constexpr int NPX = NX * NY;            // 256 pixels per tile
+  constexpr int NGROUP = NPX / 16;        // 16 pixel groups
+  constexpr int PSL = 4 / NSL;            // pixel slices
+  __shared__ float tile[2][G::TH * G::TW];
+  __shared__ __attribute__((aligned(16))) T xhi[NPX * XS];
+  __shared__ __attribute__((aligned(16))) T xlo[M::LO_ROWS ? NPX * XS : 16];
+  __shared__ float part[NSL > 1 ? NSL : 1][NSL > 1 ? NPX : 1][2];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nslice = wave % NSL;
+  const int pslice = wave / NSL;
+  const int row = lane & 15, quad = lane >> 4;
+
+  // this wave's weight fragments and the accumulator initial values (scaled biases)
+  typename M::W w1[TPW], w2[TPW];
+  float4v b1[TPW], b2[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int n = (nslice * TPW + t) * 16 + row;          // neuron of this lane's A row
+    M::load_w(w1[t], w_hi, w_lo, wsc, n, 0, quad);
+    M::load_w(w2[t], w_hi, w_lo, wsc, n, 1, quad);
     // D rows (neurons) of this lane: 4*quad + i
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -269,7 +370,33 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   // accumulator scale: exp2 argument acc*inv_acc, and 1 + |t| -> acc_one + |acc|
   const float acc_one = SCALED ? 1.0f / inv_acc : 1.0f;
 
-  // The halo of the block's next tile is loaded into registers while this one computes.
+  // softmax-weighted sums of one pixel group over this wave's neuron tiles; on return
+  // every lane holds the sums of pixel (g*16 + row)
+  auto group_sums = [&](int g, float& ws, float& vs) {
+    typename M::B bx;
+    M::load_b(bx, xhi, xlo, g * 16 + row, quad);  // B column of this lane: pixel g*16 + row
+    ws = 0.f;
+    vs = 0.f;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const float4v a1 = M::contract(w1[t], bx, b1[t]);
+      const float4v a2 = M::contract(w2[t], bx, b2[t]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float e = __builtin_amdgcn_exp2f(SCALED ? a1[i] * inv_acc : a1[i]);
+        // t/(1+|t|) with t = a2/acc_one  ==  a2/(acc_one + |a2|)
+        const float q = a2[i] * __builtin_amdgcn_rcpf(acc_one + fabsf(a2[i]));
+        ws += e;
+        vs = fmaf(e, q, vs);
+      }
+    }
+    // lanes l, l^16, l^32, l^48 hold the same pixel
+    ws += __shfl_xor(ws, 16);
+    vs += __shfl_xor(vs, 16);
+    ws += __shfl_xor(ws, 32);
+    vs += __shfl_xor(vs, 32);
+  };
+
   const int per_frame = tiles_x * tiles_y;
   NTile<WH, AXIS> ht;
   auto issue = [&](int id) {
@@ -277,86 +404,74 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     ht.issue(frame_src(p, ff), p.src_dt, p.src_pitch, 0, p.W, p.H, (rr % tiles_x) * NX + G::OX,
              (rr / tiles_x) * NY + G::OY);
   };
-  if (static_cast<int>(blockIdx.x) < total_tiles) issue(blockIdx.x);
-  for (int tileid = blockIdx.x; tileid < total_tiles; tileid += gridDim.x) {
+  // launch-order index L = blockIdx + i*gridDim walks [0, total); tile = xcd_tile(L)
+  if (static_cast<int>(blockIdx.x) < total_tiles) issue(xcd_tile(blockIdx.x, total_tiles));
+  int buf = 0;
+  for (int L = blockIdx.x; L < total_tiles; L += gridDim.x, buf ^= 1) {
+    const int tileid = xcd_tile(L, total_tiles);
     const int f = tileid / per_frame;
     const int rem = tileid - f * per_frame;
     const int y0 = (rem / tiles_x) * NY, x0 = (rem % tiles_x) * NX;
-    __syncthreads();  // previous iteration done with the LDS
-    ht.commit(tile, p.src_dt, p.src_max);
+    // The other buffer was last read two tiles ago, before the barrier below was last
+    // passed by every wave.
+    ht.commit(tile[buf], p.src_dt, p.src_max);
     __syncthreads();
-    if (tileid + static_cast<int>(gridDim.x) < total_tiles) issue(tileid + gridDim.x);
+    if (L + static_cast<int>(gridDim.x) < total_tiles) issue(xcd_tile(L + gridDim.x, total_tiles));
 
     // one thread per pixel: normalised window -> 16-bit rows (hi, lo), zero padded to KP
     const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;
     float m, sd;
     float orig;
     {
+      const float* tl = tile[buf];
       float xv[K];
 #pragma unroll
       for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int b = 0; b < WH; ++b) xv[a * WH + b] = tile[G::at(tx, ty, a, b)];
-      orig = tile[G::centre(tx, ty)];
+        for (int b = 0; b < WH; ++b) xv[a * WH + b] = tl[G::at(tx, ty, a, b)];
+      orig = tl[G::centre(tx, ty)];
       normalise<K>(xv, m, sd);
-      T* rh = xhi + threadIdx.x * XS;
-      T* rl = xlo + (TERMS >= 3 ? threadIdx.x * XS : 0);
-#pragma unroll
-      for (int k = 0; k < KP; ++k) {
-        const float v = k < K ? (SCALED ? xv[k] * xscale : xv[k]) : 0.f;
-        const T h = O::cvt(v);
-        rh[k] = h;
-        if (TERMS >= 3) rl[k] = O::cvt(v - O::back(h));
-      }
+      M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED);
     }
-    __syncthreads();
 
-    // MFMA over this wave's pixel groups and neuron tiles
-    for (int g = pslice; g < NGROUP; g += PSL) {
-      const int px = g * 16 + row;  // B column of this lane
-      typename O::V8 bh[KC], bl[KC];
+    float wsum = 0.f, vsum = 0.f;
+    if constexpr (NSL == 1) {
+      // this wave's own rows: groups 4*wave + j; after group j, the lanes of quad j hold
+      // their own pixel's sums
+      wave_lds_sync();
 #pragma unroll
-      for (int c = 0; c < KC; ++c) {
-        bh[c] = *reinterpret_cast<const typename O::V8*>(xhi + px * XS + 32 * c + 8 * quad);
-        if (TERMS >= 3) bl[c] = *reinterpret_cast<const typename O::V8*>(xlo + px * XS + 32 * c + 8 * quad);
-      }
-      float ws = 0.f, vs = 0.f;
-#pragma unroll
-      for (int t = 0; t < TPW; ++t) {
-        const float4v a1 = contract<KC, IS_BF16, TERMS>(w1[t], bh, bl, b1[t]);
-        const float4v a2 = contract<KC, IS_BF16, TERMS>(w2[t], bh, bl, b2[t]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float e = __builtin_amdgcn_exp2f(SCALED ? a1[i] * inv_acc : a1[i]);
-          // t/(1+|t|) with t = a2/acc_one  ==  a2/(acc_one + |a2|)
-          const float q = a2[i] * __builtin_amdgcn_rcpf(acc_one + fabsf(a2[i]));
-          ws += e;
-          vs = fmaf(e, q, vs);
+      for (int j = 0; j < 4; ++j) {
+        float ws, vs;
+        group_sums(wave * 4 + j, ws, vs);
+        if (quad == j) {
+          wsum = ws;
+          vsum = vs;
         }
       }
-      // lanes l, l^16, l^32, l^48 hold the same pixel
-      ws += __shfl_xor(ws, 16);
-      vs += __shfl_xor(vs, 16);
-      ws += __shfl_xor(ws, 32);
-      vs += __shfl_xor(vs, 32);
-      if (quad == 0) {
-        part[nslice][px][0] = ws;
-        part[nslice][px][1] = vs;
+    } else {
+      __syncthreads();
+      for (int g = pslice; g < NGROUP; g += PSL) {
+        float ws, vs;
+        group_sums(g, ws, vs);
+        if (quad == 0) {
+          part[nslice][g * 16 + row][0] = ws;
+          part[nslice][g * 16 + row][1] = vs;
+        }
       }
-    }
-    __syncthreads();
-
-    const int x = x0 + tx, y = y0 + ty;
-    if (x < p.W && y < p.H) {
-      float wsum = 0.f, vsum = 0.f;
+      __syncthreads();
 #pragma unroll
       for (int s = 0; s < NSL; ++s) {
         wsum += part[s][threadIdx.x][0];
         vsum += part[s][threadIdx.x][1];
       }
+    }
+
+    const int x = x0 + tx, y = y0 + ty;
+    if (x < p.W && y < p.H) {
       const float pred = fminf(fmaxf(m + 5.0f * vsum / wsum * sd, 0.f), 1.f);
       store_result<AXIS>(p, frame_dst(p, f), x, y, orig, pred);
     }
+    if constexpr (NSL == 1) wave_lds_sync();  // this tile's row reads before the next tile's writes
   }
 }
 
@@ -377,7 +492,7 @@ template <int WH, int AXIS>
 static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t st) {
   const int tiles_x = (p.W + NX - 1) / NX, tiles_y = (p.H + NY - 1) / NY;
   if (nw.precision == 0) {
-    dim3 grid(tiles_x, tiles_y, p.batch);
+    dim3 grid(tiles_x * tiles_y * p.batch);
     hipLaunchKernelGGL((nnedi3_valu<WH, AXIS>), grid, dim3(256), 0, st, p, static_cast<const float*>(nw.w), nw.nns);
     return hipGetLastError();
   }
@@ -387,31 +502,34 @@ static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t s
   // Neuron tiles are split over NSL waves only when one wave cannot hold them all
   // (nns >= 128); otherwise every wave owns all neurons for its own pixel groups.
   // Persistent grid: as many blocks as the occupancy of the instance allows per CU.
-#define MPX_NN(BF, TERMS, NSL, TPW, SC)                                                               \
+#define MPX_NN(FM, TERMS, NSL, TPW, SC)                                                               \
   {                                                                                                   \
-    auto kern = nnedi3_mfma<WH, AXIS, BF, TERMS, NSL, TPW, SC>;                                       \
-    int per_cu = 0;                                                                                   \
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) \
+    auto kern = nnedi3_mfma<WH, AXIS, FM, TERMS, NSL, TPW, SC>;                                       \
+    static int per_cu = 0; /* per instance, queried once (no runtime queries while capturing) */      \
+    if (!per_cu &&                                                                                    \
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1)) \
       per_cu = 1;                                                                                     \
     const int grid = min(total, cu_count() * per_cu);                                                 \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.bias, nw.xscale,       \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
                        inv_acc, tiles_x, tiles_y, total);                                             \
   }
-#define MPX_NN_NT(BF, TERMS, SC)                    \
+#define MPX_NN_NT(FM, TERMS, SC)                    \
   switch (nt) {                                     \
-    case 1: MPX_NN(BF, TERMS, 1, 1, SC) break;      \
-    case 2: MPX_NN(BF, TERMS, 1, 2, SC) break;      \
-    case 4: MPX_NN(BF, TERMS, 1, 4, SC) break;      \
-    case 8: MPX_NN(BF, TERMS, 2, 4, SC) break;      \
-    case 16: MPX_NN(BF, TERMS, 4, 4, SC) break;     \
+    case 1: MPX_NN(FM, TERMS, 1, 1, SC) break;      \
+    case 2: MPX_NN(FM, TERMS, 1, 2, SC) break;      \
+    case 4: MPX_NN(FM, TERMS, 1, 4, SC) break;      \
+    case 8: MPX_NN(FM, TERMS, 2, 4, SC) break;      \
+    case 16: MPX_NN(FM, TERMS, 4, 4, SC) break;     \
     default: return hipErrorInvalidValue;           \
   }
   if (nw.precision == 1) {
-    MPX_NN_NT(false, 1, false)
+    MPX_NN_NT(F_F16, 1, false)
   } else if (nw.precision == 4) {
-    MPX_NN_NT(false, 3, true)
+    MPX_NN_NT(F_F16, 3, true)
   } else if (nw.precision == 2) {
-    MPX_NN_NT(true, 3, false)
+    MPX_NN_NT(F_BF16, 3, false)
+  } else if (nw.precision == 3) {
+    MPX_NN_NT(F_FP8, 1, false)
   } else {
     return hipErrorInvalidValue;
   }

@@ -1,4 +1,5 @@
 // NNEDI3 weight packing per precision (host side), uploaded once per context.
+#include <algorithm>
 #include <cmath>
 
 #include "internal.h"
@@ -23,6 +24,27 @@ float bf16_float(uint16_t b) {
   float f;
   std::memcpy(&f, &u, 4);
   return f;
+}
+
+// Round-to-nearest-even double -> OCP FP8 e4m3 (bias 7, largest finite 448, saturating).
+uint8_t e4m3_bits(double v) {
+  const uint8_t sign = v < 0 ? 0x80 : 0;
+  const double a = std::fabs(v);
+  if (!(a < 448.0)) return sign | 0x7e;
+  int e;
+  std::frexp(a, &e);                       // a = f * 2^e, f in [0.5, 1)
+  int E = e - 1 + 7;                       // biased exponent of a normal encoding
+  if (E < 1) {                             // subnormal: multiples of 2^-9
+    const double mm = std::nearbyint(a * 512.0);
+    return sign | static_cast<uint8_t>(mm >= 8 ? 0x08 : static_cast<int>(mm));
+  }
+  double mm = std::nearbyint((a / std::ldexp(1.0, e - 1) - 1.0) * 8.0);
+  if (mm >= 8) {
+    mm = 0;
+    ++E;
+  }
+  if (E > 15 || (E == 15 && mm > 6)) return sign | 0x7e;
+  return sign | static_cast<uint8_t>((E << 3) | static_cast<int>(mm));
 }
 
 }  // namespace
@@ -107,7 +129,31 @@ int nnedi3_prepare(mpx_ctx* ctx, int nns, int wh, int precision, NnediWeights* o
     out->w_lo = p;
     return MPX_OK;
   }
-  return fail(MPX_ERR_UNSUPPORTED, "NNEDI3 precision " + std::to_string(precision) + " not built yet");
+  if (precision == MPX_PREC_FP8) {
+    std::vector<uint8_t> q(static_cast<size_t>(nns) * 2 * 128, 0);
+    std::vector<int> ex(static_cast<size_t>(nns) * 2);
+    for (int n = 0; n < nns; ++n)
+      for (int mtx = 0; mtx < 2; ++mtx) {
+        double mx = 0;
+        for (int k = 0; k < K; ++k) mx = std::max(mx, std::fabs(mtx == 0 ? W1(n, k) : W2(n, k)));
+        int e = 0;
+        if (mx > 0) {
+          std::frexp(mx / 448.0, &e);  // mx / 448 = f * 2^e, f in [0.5, 1): mx * 2^-e in [224, 448)
+          if (std::ldexp(mx, -e) > 448.0) ++e;
+        }
+        ex[n * 2 + mtx] = 127 + e;
+        uint8_t* r = &q[(static_cast<size_t>(n) * 2 + mtx) * 128];
+        for (int k = 0; k < K; ++k) r[k] = r[K + k] = e4m3_bits(std::ldexp(mtx == 0 ? W1(n, k) : W2(n, k), -e));
+      }
+    rc = ctx->upload(name + "/fp8", q.data(), q.size(), &p);
+    if (rc) return rc;
+    out->w = p;
+    rc = ctx->upload(name + "/fp8_exp", ex.data(), ex.size() * 4, &p);
+    if (rc) return rc;
+    out->wexp = static_cast<const int*>(p);
+    return MPX_OK;
+  }
+  return fail(MPX_ERR_UNSUPPORTED, "NNEDI3 precision " + std::to_string(precision) + " not built");
 }
 
 }  // namespace mpx

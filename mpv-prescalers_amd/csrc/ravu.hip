@@ -58,8 +58,9 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   constexpr int C = (N * N - 1) / 2;
   constexpr int NTEX = 2 * (C + 1);
   __shared__ float tile[P * PN];
-  const int f = blockIdx.z;
-  const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
+  const BlockXYZ bk3 = block_xyz((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
+  const int f = bk3.z;
+  const int x0 = bk3.x * RT_X, y0 = bk3.y * RT_Y;
   const void* in = frame_in(io, f);
   stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - HALO, y0 - HALO);
   __syncthreads();
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
 }
 
 hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, hipStream_t st) {
-  dim3 grid((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
+  dim3 grid(((io.W + RT_X - 1) / RT_X) * ((io.H + RT_Y - 1) / RT_Y) * io.batch);
 #define MPX_3X(R, CH)                                                                                  \
   if (r == R && channels == CH) {                                                                      \
     hipLaunchKernelGGL((ravu_3x_kernel<R, CH>), grid, dim3(256), 0, st, io, lut);                      \
@@ -167,8 +168,9 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   constexpr int N = 2 * R;
   constexpr int PW = RT_X + N - 1, PH = RT_Y + N - 1, PN = PW * PH;
   __shared__ float tile[P * PN];
-  const int f = blockIdx.z;
-  const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
+  const BlockXYZ bk3 = block_xyz((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
+  const int f = bk3.z;
+  const int x0 = bk3.x * RT_X, y0 = bk3.y * RT_Y;
   const void* in = frame_in(io, f);
   stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - (R - 1),
                   y0 - (R - 1));
@@ -210,8 +212,9 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
   constexpr int PW = RT_X + 2 * R + 1, PH = RT_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
   __shared__ float hk[P * PN];   // input (HOOKED)
   __shared__ float i11[P * PN];  // ravu_int11
-  const int f = blockIdx.z;
-  const int x0 = blockIdx.x * RT_X, y0 = blockIdx.y * RT_Y;
+  const BlockXYZ bk3 = block_xyz((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
+  const int f = bk3.z;
+  const int x0 = bk3.x * RT_X, y0 = bk3.y * RT_Y;
   const void* in = frame_in(io, f);
   const int64_t plane = static_cast<int64_t>(io.W) * io.H;
   {
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
 
 hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
                        hipStream_t st) {
-  dim3 grid((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
+  dim3 grid(((io.W + RT_X - 1) / RT_X) * ((io.H + RT_Y - 1) / RT_Y) * io.batch);
 #define MPX_RAVU(R, CH)                                                                                        \
   if (r == R && channels == CH) {                                                                              \
     hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);       \

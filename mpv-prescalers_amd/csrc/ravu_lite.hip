@@ -28,9 +28,10 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   constexpr int C = (N * N - 1) / 2;
   __shared__ float tile[PH * PW];
 
-  const int frame = blockIdx.z;
-  const int x0 = blockIdx.x * LITE_TX;
-  const int y0 = blockIdx.y * LITE_TY;
+  const BlockXYZ bk3 = block_xyz((io.W + LITE_TX - 1) / LITE_TX, (io.H + LITE_TY - 1) / LITE_TY, io.batch);
+  const int frame = bk3.z;
+  const int x0 = bk3.x * LITE_TX;
+  const int y0 = bk3.y * LITE_TY;
   const void* in = frame_in(io, frame);
 
   {
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
 }
 
 hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, hipStream_t st) {
-  dim3 grid((io.W + LITE_TX - 1) / LITE_TX, (io.H + LITE_TY - 1) / LITE_TY, io.batch);
+  dim3 grid(((io.W + LITE_TX - 1) / LITE_TX) * ((io.H + LITE_TY - 1) / LITE_TY) * io.batch);
   dim3 block(LITE_TX * LITE_TY);
 #define MPX_LITE(R, A) \
   if (r == R && ar == A) { hipLaunchKernelGGL((ravu_lite_kernel<R, A>), grid, block, 0, st, io, lut); return hipGetLastError(); }

@@ -94,8 +94,9 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   __shared__ float tile[P * PN];
   __shared__ int bucket[ZB_Y * ZB_X];
 
-  const int f = blockIdx.z;
-  const int ox0 = blockIdx.x * ZT_X, oy0 = blockIdx.y * ZT_Y;
+  const BlockXYZ bk3 = block_xyz((out_w + ZT_X - 1) / ZT_X, (out_h + ZT_Y - 1) / ZT_Y, io.batch);
+  const int f = bk3.z;
+  const int ox0 = bk3.x * ZT_X, oy0 = bk3.y * ZT_Y;
   const int bx0 = phase_of(ox0, io.W, out_w).base;
   const int by0 = phase_of(oy0, io.H, out_h).base;
   const void* in = frame_in(io, f);
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
 hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
                             const ZoomPlan& zp, int out_w, int out_h, hipStream_t st) {
   (void)zp;
-  dim3 grid((out_w + ZT_X - 1) / ZT_X, (out_h + ZT_Y - 1) / ZT_Y, io.batch);
+  dim3 grid(((out_w + ZT_X - 1) / ZT_X) * ((out_h + ZT_Y - 1) / ZT_Y) * io.batch);
 #define MPX_ZOOM(R, A, CH)                                                                             \
   if (r == R && ar == A && channels == CH) {                                                           \
     hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH>), grid, dim3(256), 0, st, io, lut, lut_ar, out_w, out_h); \

@@ -92,3 +92,28 @@ def test_nnedi3_flat_and_tiny():
             out = mpx.upscale("nnedi3-nns32-win8x6", torch.from_numpy(img).cuda(), precision=prec).cpu().numpy()
             ref = nnedi3(img.astype(np.float64), 32, 6)
             assert np.abs(out - ref).max() <= TOL_MFMA, (w, h, prec)
+
+
+@pytest.mark.parametrize("nns,wh", [(16, 4), (32, 4), (64, 6), (128, 6), (256, 6)])
+def test_nnedi3_fp8_form(nns, wh):
+    """FP8 e4m3 form (per-neuron power-of-two scales).  One pass on exact 10-bit inputs
+    equals the NumPy emulation of the form to 2e-4 (FP32 vs FP64 normalisation can flip an
+    e4m3 rounding tie of a window value; its low part absorbs most of the change).  Both
+    passes: the second pass quantises the first pass's FP32 output, so the two
+    implementations' small first-pass differences can move e4m3 roundings there: bound
+    2e-3.  The form's own error vs the FP64 oracle is reported and bounded by 0.06."""
+    import mpx
+    from fp8emu import nnedi3_fp8
+    from oracle.nnedi3 import nnedi3
+    name = "nnedi3-nns%d-win8x%d" % (nns, wh)
+    img = _frame(96, 64, seed=300 + nns, bits=10)
+    x = torch.from_numpy(img).cuda()
+    y_only = mpx.upscale(name, x, target=(96, 128), precision="fp8").cpu().numpy()
+    f = compare("%s fp8 double_y vs emulation 96x64" % name, y_only, nnedi3_fp8(img, nns, wh, axes="y"))
+    assert f["max_abs"] <= 2e-4
+    got = mpx.upscale(name, x, precision="fp8").cpu().numpy()
+    f = compare("%s fp8 vs emulation 96x64" % name, got, nnedi3_fp8(img, nns, wh))
+    assert f["max_abs"] <= 2e-3
+    g = compare("%s fp8 vs FP64 oracle 96x64" % name, got, nnedi3(img.astype(np.float64), nns, wh))
+    assert g["max_abs"] <= 0.06
+    assert np.array_equal(got[0::2, 0::2], img)
