@@ -39,7 +39,7 @@ extern "C" {
 #define MPX_PREC_FP32 0   /* plain FP32 VALU kernel                                       */
 #define MPX_PREC_FP16 1   /* FP16 MFMA, one product, FP32 accumulation                    */
 #define MPX_PREC_BF16 2   /* BF16 MFMA, high/low split (three products), FP32 accumulation */
-#define MPX_PREC_FP8 3    /* FP8 e4m3 MFMA, per-neuron FP32 scales                        */
+#define MPX_PREC_FP8 3    /* FP8 e4m3 MFMA, per-neuron power-of-two scales, FP32 accum.   */
 #define MPX_PREC_FP16X3 4 /* FP16 MFMA, high/low split (three products), FP32 accumulation */
 
 /* ---- variant families / channel forms --------------------------------------------- */
@@ -141,6 +141,23 @@ int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream);
 #define MPX_MAX_FRAMES 16
 int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* outs, int64_t in_pitch,
                    int64_t out_pitch, void* hip_stream);
+
+/* ---- the scaling pass that consumes a prescaler's OFFSET ---------------------------- */
+#define MPX_FILTER_BILINEAR 0
+#define MPX_FILTER_CATMULL_ROM 1
+#define MPX_FILTER_MITCHELL 2
+#define MPX_FILTER_SPLINE36 3
+#define MPX_FILTER_LANCZOS3 4
+
+/* Resample `planes` planes per frame of in_w x in_h (io->batch frames, strides as in
+ * mpx_io) to out_w x out_h with a separable filter, sampling the input at
+ * (j + 1/2) * in/out + offset per axis: pass the prescaler's mpx_variant_info.offset_x/y
+ * to get a display-aligned image.  Replaces mpv's main scaler pass, which applies the
+ * accumulated `//!OFFSET` of the hooks before it (e.g. nnedi3-nns256-win8x6.hook:352,
+ * ravu-r3.hook:426-428).  Types MPX_U8/U16 (value/(2^bits-1)), F16, F32 in; F16/F32 out.
+ * Allocates nothing; asynchronous on hip_stream. */
+int mpx_resample(const mpx_io* io, int in_w, int in_h, int in_type, int bits, int out_w, int out_h,
+                 int out_type, int planes, int filter, float offset_x, float offset_y, void* hip_stream);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char* mpx_last_error(void);

@@ -405,3 +405,43 @@ int mpx_run(mpx_plan* plan, const void* in, int64_t in_pitch, int64_t in_bstride
 const char* mpx_last_error(void) { return mpx_host::last_error().c_str(); }
 
 }  // extern "C"
+
+int mpx_resample(const mpx_io* io, int in_w, int in_h, int in_type, int bits, int out_w, int out_h, int out_type,
+                 int planes, int filter, float offset_x, float offset_y, void* hip_stream) {
+  if (!io || !io->in || !io->out) return fail(MPX_ERR_ARG, "io/in/out is NULL");
+  if (filter < MPX_FILTER_BILINEAR || filter > MPX_FILTER_LANCZOS3) return fail(MPX_ERR_ARG, "bad filter");
+  if (in_type < MPX_U8 || in_type > MPX_F32 || (out_type != MPX_F16 && out_type != MPX_F32))
+    return fail(MPX_ERR_ARG, "bad element type");
+  if (in_type == MPX_U16 && (bits < 9 || bits > 16)) return fail(MPX_ERR_ARG, "bits outside 9..16");
+  if (planes != 1 && planes != 3) return fail(MPX_ERR_ARG, "planes must be 1 or 3");
+  if (in_w < 1 || in_h < 1 || out_w < 1 || out_h < 1 || in_w > 32768 || in_h > 32768 || out_w > 32768 ||
+      out_h > 32768)
+    return fail(MPX_ERR_SHAPE, "sizes outside 1..32768");
+  if (!std::isfinite(offset_x) || !std::isfinite(offset_y) || std::fabs(offset_x) > 64.f || std::fabs(offset_y) > 64.f)
+    return fail(MPX_ERR_ARG, "offset outside [-64, 64]");
+  if (io->batch < 1 || io->batch > 65535) return fail(MPX_ERR_SHAPE, "batch outside 1..65535");
+  if (io->in_pitch < in_w || io->out_pitch < out_w) return fail(MPX_ERR_ARG, "pitch smaller than the row width");
+  if (planes > 1 && (io->in_cstride < io->in_pitch * in_h || io->out_cstride < io->out_pitch * out_h))
+    return fail(MPX_ERR_ARG, "plane stride smaller than a plane");
+  if (io->batch > 1 &&
+      (io->in_bstride < io->in_pitch * in_h * planes || io->out_bstride < io->out_pitch * out_h * planes))
+    return fail(MPX_ERR_ARG, "frame stride smaller than a frame");
+  mpx::Io d{};
+  d.in = io->in;
+  d.out = io->out;
+  d.in_pitch = io->in_pitch;
+  d.in_cstride = io->in_cstride;
+  d.in_bstride = io->in_bstride;
+  d.out_pitch = io->out_pitch;
+  d.out_cstride = io->out_cstride;
+  d.out_bstride = io->out_bstride;
+  d.in_dt = in_type;
+  d.out_dt = out_type;
+  d.in_max = in_type == MPX_U8 ? 255.f : (in_type == MPX_U16 ? static_cast<float>((1 << bits) - 1) : 1.f);
+  d.W = in_w;
+  d.H = in_h;
+  d.batch = io->batch;
+  HIP_TRY(mpx::launch_resample(d, out_w, out_h, planes, filter, offset_x, offset_y,
+                               static_cast<hipStream_t>(hip_stream)));
+  return MPX_OK;
+}

@@ -167,8 +167,15 @@ __device__ __forceinline__ T deriv(Line s, int p) {
 #pragma clang fp contract(off)
   if (p == 0) return s(1) - s(0);
   if (p == N - 1) return s(N - 1) - s(N - 2);
-  if (FIVE && p >= 2 && p + 2 <= N - 1)
-    return (-s(p + 2) + T(8) * s(p + 1) - T(8) * s(p - 1) + s(p - 2)) / T(12);
+  // FP32 (the fast path, whose decisions are trusted only away from bucket boundaries)
+  // multiplies by the reciprocals: exact for 1/2, within one rounding of the quotient for
+  // 1/12.  FP64 (the re-decision) divides, as the oracle does.
+  if (FIVE && p >= 2 && p + 2 <= N - 1) {
+    const T num = -s(p + 2) + T(8) * s(p + 1) - T(8) * s(p - 1) + s(p - 2);
+    if constexpr (sizeof(T) == 4) return num * T(1.0 / 12.0);
+    return num / T(12);
+  }
+  if constexpr (sizeof(T) == 4) return (s(p + 1) - s(p - 1)) * T(0.5);
   return (s(p + 1) - s(p - 1)) / T(2);
 }
 

@@ -14,4 +14,8 @@ hipError_t launch_copy(const Io& io, int planes, hipStream_t st);
 // RAVU-lite(-ar), r in {2,3,4}; lut = FP16 [288][ceil(n^2/2)][4].
 hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, hipStream_t st);
 
+// Separable resampler (filter MPX_FILTER_*), io.W x io.H -> out_w x out_h, with offsets.
+hipError_t launch_resample(const Io& io, int out_w, int out_h, int planes, int filter, float off_x, float off_y,
+                           hipStream_t st);
+
 }  // namespace mpx

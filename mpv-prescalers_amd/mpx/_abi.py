@@ -17,13 +17,14 @@ STATUS = {
 }
 U8, U16, F16, F32 = 0, 1, 2, 3
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "fp8": 3, "fp16x3": 4}
+FILTER = {"bilinear": 0, "catmull_rom": 1, "mitchell": 2, "spline36": 3, "lanczos3": 4}
 FAMILY = {0: "nnedi3", 1: "ravu", 2: "ravu-lite", 3: "ravu-3x", 4: "ravu-zoom"}
 CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 
 # every exported symbol declared in include/mpx.h
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
-           "mpx_run_ex", "mpx_run_frames", "mpx_last_error")
+           "mpx_run_ex", "mpx_run_frames", "mpx_resample", "mpx_last_error")
 MAX_FRAMES = 16
 
 
@@ -75,6 +76,8 @@ def _load():
         "mpx_run": (i32, [vp, vp, i64, i64, vp, i64, i64, i32, vp]),
         "mpx_run_ex": (i32, [vp, ctypes.POINTER(MpxIo), vp]),
         "mpx_run_frames": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), i64, i64, vp]),
+        "mpx_resample": (i32, [ctypes.POINTER(MpxIo), i32, i32, i32, i32, i32, i32, i32, i32, i32,
+                               ctypes.c_float, ctypes.c_float, vp]),
         "mpx_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

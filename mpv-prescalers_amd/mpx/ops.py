@@ -218,3 +218,45 @@ def upscale(name, x, target=None, precision="fp32", out_dtype=None, bits=10, buc
     tgt = tuple(target) if target is not None else default_target(name, in_w, in_h)
     plan = default_context(x.device).cached_plan(name, in_w, in_h, tgt, x.dtype, out_dtype, precision, batch, bits)
     return plan.run(x, stream=stream, buckets=buckets)
+
+
+def resample(x, size, offset=(0.0, 0.0), filter="catmull_rom", planes=1, out_dtype=None, bits=10, stream=None):
+    """The scaling pass after a prescaler: resample device tensor ``x`` to ``size`` =
+    (width, height), sampling at (j + 1/2) * in/out + offset per axis.  ``planes`` = 1:
+    ``x`` is (H, W) or (B, H, W); 3: (3, H, W) or (B, 3, H, W).  Pass the prescaler's
+    ``info["offset_x"], info["offset_y"]`` to obtain the display-aligned image (what mpv's
+    main scaler does with a hook's OFFSET)."""
+    return _resample(x, size, offset, filter, out_dtype, bits, stream, planes)
+
+
+def _resample(x, size, offset, filter, out_dtype, bits, stream, planes):
+    if not x.is_cuda:
+        raise ValueError("input must be a device tensor")
+    if x.stride(-1) != 1:
+        x = x.contiguous()
+    out_w, out_h = size
+    in_h, in_w = x.shape[-2:]
+    nd = 2 if planes == 1 else 3
+    if x.dim() not in (nd, nd + 1) or (planes == 3 and x.shape[-3] != 3):
+        raise ValueError("bad input shape %s for %d plane(s)" % (tuple(x.shape), planes))
+    batch = x.shape[0] if x.dim() == nd + 1 else 1
+    if out_dtype is None:
+        out_dtype = x.dtype if x.dtype in (torch.float16, torch.float32) else torch.float32
+    out = torch.empty(tuple(x.shape[:-2]) + (out_h, out_w), dtype=out_dtype, device=x.device)
+    io = MpxIo()
+    io.in_ = x.data_ptr()
+    io.in_pitch = x.stride(-2)
+    io.in_cstride = x.stride(-3) if planes == 3 else x.stride(-2) * in_h
+    io.in_bstride = x.stride(0) if x.dim() == nd + 1 else x.numel()
+    io.out = out.data_ptr()
+    io.out_pitch = out.stride(-2)
+    io.out_cstride = out.stride(-3) if planes == 3 else out.stride(-2) * out_h
+    io.out_bstride = out.stride(0) if x.dim() == nd + 1 else out.numel()
+    io.batch = batch
+    if stream is None:
+        stream = torch.cuda.current_stream(x.device)
+    with torch.cuda.device(x.device):
+        check(LIB.mpx_resample(ctypes.byref(io), in_w, in_h, _DT[x.dtype], bits, out_w, out_h, _DT[out_dtype], planes,
+                               _abi.FILTER[filter], float(offset[0]), float(offset[1]),
+                               ctypes.c_void_p(stream.cuda_stream)))
+    return out

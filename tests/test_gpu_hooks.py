@@ -1,0 +1,50 @@
+"""Hook-point routing, WHEN gating and presentation (mpx.hooks) on the GPU."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _yuv(w, h, sub=2, seed=0):
+    from mpx.frames import make_plane
+    y = torch.from_numpy(make_plane(w, h, seed, bits=10).astype(np.float32)).cuda()
+    cw, ch = -(-w // sub), -(-h // sub)
+    u = torch.from_numpy(make_plane(cw, ch, seed + 1, bits=10).astype(np.float32)).cuda()
+    v = torch.from_numpy(make_plane(cw, ch, seed + 2, bits=10).astype(np.float32)).cuda()
+    return {"Y": y, "U": u, "V": v}
+
+
+def test_luma_hook_on_420():
+    import mpx
+    from mpx.hooks import apply_hook, present
+    fr = _yuv(64, 36)
+    out, offs, ran = apply_hook("ravu-r3", fr, (128, 72))
+    assert ran and out["U"] is fr["U"] and out["V"] is fr["V"]
+    assert torch.equal(out["Y"], mpx.upscale("ravu-r3", fr["Y"]))
+    assert offs["Y"] == (-0.5, -0.5) and offs["U"] == (0.0, 0.0)
+    shown = present(out, offs, (128, 72))
+    assert all(tuple(p.shape) == (72, 128) for p in shown.values())
+
+
+def test_when_false_passes_through():
+    from mpx.hooks import apply_hook
+    fr = _yuv(64, 36)
+    out, offs, ran = apply_hook("nnedi3-nns32-win8x4", fr, (80, 45))   # 1.25x: WHEN false
+    assert not ran and out["Y"] is fr["Y"]
+
+
+def test_native_and_main_routing():
+    import mpx
+    from mpx.hooks import apply_hook
+    fr444 = _yuv(48, 32, sub=1, seed=5)
+    out, offs, ran = apply_hook("ravu-r2-yuv", fr444, (96, 64))
+    want = mpx.upscale("ravu-r2-yuv", torch.stack([fr444["Y"], fr444["U"], fr444["V"]]))
+    assert ran and torch.equal(out["V"], want[2])
+    with pytest.raises(ValueError):
+        apply_hook("ravu-r2-yuv", _yuv(48, 32), (96, 64))                 # 4:2:0 into NATIVE
+    rgb = {"RGB": torch.stack([fr444["Y"], fr444["U"], fr444["V"]])}
+    out, offs, ran = apply_hook("ravu-zoom-ar-r2-rgb", rgb, (72, 48))
+    assert ran and tuple(out["RGB"].shape) == (3, 48, 72)
+    out, offs, ran = apply_hook("ravu-lite-r2", rgb, (96, 64))             # no LUMA in RGB
+    assert not ran

@@ -97,8 +97,9 @@ def test_nnedi3_flat_and_tiny():
 @pytest.mark.parametrize("nns,wh", [(16, 4), (32, 4), (64, 6), (128, 6), (256, 6)])
 def test_nnedi3_fp8_form(nns, wh):
     """FP8 e4m3 form (per-neuron power-of-two scales).  One pass on exact 10-bit inputs
-    equals the NumPy emulation of the form to 2e-4 (FP32 vs FP64 normalisation can flip an
-    e4m3 rounding tie of a window value; its low part absorbs most of the change).  Both
+    equals the NumPy emulation of the form to 5e-4 max-abs and >= 100 dB PSNR (FP32 vs
+    FP64 normalisation can move a window value across an e4m3 rounding boundary; its low
+    part absorbs most of the change).  Both
     passes: the second pass quantises the first pass's FP32 output, so the two
     implementations' small first-pass differences can move e4m3 roundings there: bound
     2e-3.  The form's own error vs the FP64 oracle is reported and bounded by 0.06."""
@@ -110,7 +111,7 @@ def test_nnedi3_fp8_form(nns, wh):
     x = torch.from_numpy(img).cuda()
     y_only = mpx.upscale(name, x, target=(96, 128), precision="fp8").cpu().numpy()
     f = compare("%s fp8 double_y vs emulation 96x64" % name, y_only, nnedi3_fp8(img, nns, wh, axes="y"))
-    assert f["max_abs"] <= 2e-4
+    assert f["max_abs"] <= 5e-4 and f["psnr"] >= 100
     got = mpx.upscale(name, x, precision="fp8").cpu().numpy()
     f = compare("%s fp8 vs emulation 96x64" % name, got, nnedi3_fp8(img, nns, wh))
     assert f["max_abs"] <= 2e-3
