@@ -142,6 +142,17 @@ int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream);
 int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* outs, int64_t in_pitch,
                    int64_t out_pitch, void* hip_stream);
 
+/* Autotune the plan's launch shape on the caller's buffers: runs the plan on `io`
+ * (warm-up + `iters` timed runs per candidate, HIP events on hip_stream, synchronising)
+ * over the candidate shapes and keeps the fastest in the plan.  NNEDI3 MFMA plans: the
+ * persistent grid (blocks per CU) and, for nns = 128, the neuron split across waves.
+ * Other plans have no tunable shape: returns MPX_OK with *grid_per_cu = 0.  Writes the
+ * output buffers.  Outputs (may be NULL): the chosen blocks per CU (0 = occupancy
+ * maximum) and split, and the best time per run in microseconds.  Replaces the fixed
+ * workgroup shape of the compute shaders (e.g. compute/ravu-3x-r3.hook:22 `//!COMPUTE`). */
+int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters, int* grid_per_cu, int* split,
+                  float* best_us);
+
 /* ---- the scaling pass that consumes a prescaler's OFFSET ---------------------------- */
 #define MPX_FILTER_BILINEAR 0
 #define MPX_FILTER_CATMULL_ROM 1

@@ -445,3 +445,61 @@ int mpx_resample(const mpx_io* io, int in_w, int in_h, int in_type, int bits, in
                                static_cast<hipStream_t>(hip_stream)));
   return MPX_OK;
 }
+
+int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters, int* grid_per_cu, int* split,
+                  float* best_us) {
+  if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
+  if (iters < 1 || iters > 1000) return fail(MPX_ERR_ARG, "iters outside 1..1000");
+  if (grid_per_cu) *grid_per_cu = 0;
+  if (split) *split = 0;
+  if (best_us) *best_us = 0.f;
+  const bool tunable = plan->v->family == MPX_FAMILY_NNEDI3 && plan->precision != MPX_PREC_FP32 && plan->info.applies;
+  if (!tunable) return MPX_OK;
+  HIP_TRY(hipSetDevice(plan->ctx->device));
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(MPX_ERR_HIP, "hipEventCreate failed");
+  }
+  const int grids[] = {0, 1, 2, 3, 4, 6, 8};
+  const int nsplit = plan->v->nns == 128 ? 2 : 1;
+  const mpx::NnediWeights saved = plan->nw;
+  float best = 0.f;
+  int best_g = 0, best_s = 0, rc = MPX_OK;
+  for (int sp = 0; sp < nsplit && rc == MPX_OK; ++sp)
+    for (int g : grids) {
+      plan->nw.grid_per_cu = g;
+      plan->nw.split = sp;
+      rc = mpx_run_ex(plan, io, hip_stream);  // warm-up; also validates io
+      if (rc != MPX_OK) break;
+      if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(MPX_ERR_HIP, "hipEventRecord failed"); break; }
+      for (int i = 0; i < iters && rc == MPX_OK; ++i) rc = mpx_run_ex(plan, io, hip_stream);
+      if (rc != MPX_OK) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(MPX_ERR_HIP, "event timing failed");
+        break;
+      }
+      const float us = ms * 1000.f / static_cast<float>(iters);
+      if (best == 0.f || us < best) {
+        best = us;
+        best_g = g;
+        best_s = sp;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != MPX_OK) {
+    plan->nw = saved;
+    return rc;
+  }
+  plan->nw.grid_per_cu = best_g;
+  plan->nw.split = best_s;
+  if (grid_per_cu) *grid_per_cu = best_g;
+  if (split) *split = best_s;
+  if (best_us) *best_us = best;
+  return MPX_OK;
+}

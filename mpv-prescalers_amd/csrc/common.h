@@ -272,17 +272,19 @@ __device__ __forceinline__ Bucket hash_fast(float a, float b, float d, bool* nea
   const float delta = sqrtf(h * h + b * b);
   const float L1 = T * 0.5f + delta;
   const float L2 = T * 0.5f - delta;
-  const float y = h <= 0.f ? delta - h : b * b / (h + delta);  // L1 - a >= 0
+  // quotients below go through the hardware reciprocal (within 1 ulp of the division):
+  // every decision they feed is re-made in FP64 when it lies within the margins
+  const float y = h <= 0.f ? delta - h : b * b * __builtin_amdgcn_rcpf(h + delta);  // L1 - a >= 0
   const float sqrtL1 = sqrtf(L1);
   const float sqrtL2 = sqrtf(L2);
   const float at = atan2f(y, b) + pi;
-  float theta = at - pi * floorf(at / pi);
+  float theta = at - pi * floorf(at * (1.0f / 3.141592653589793f));
   const bool bsmall = fabsf(b) < eps;
   if (bsmall) theta = 0.f;
   const float ssum = sqrtL1 + sqrtL2;
-  float mu = (sqrtL1 - sqrtL2) / ssum;
+  float mu = (sqrtL1 - sqrtL2) * __builtin_amdgcn_rcpf(ssum);
   if (ssum < eps) mu = 0.f;
-  const float u = theta * 24.0f / pi;
+  const float u = theta * static_cast<float>(24.0 / 3.141592653589793);
   Bucket bk;
   bk.angle = static_cast<int>(floorf(u));
   bk.strength = strength_of<RULE, float>(sqrtL1);
@@ -291,7 +293,7 @@ __device__ __forceinline__ Bucket hash_fast(float a, float b, float d, bool* nea
   bool n = fabsf(fabsf(b) - eps) <= rel * eps + ulp16 * T;
   n |= fabsf(L2) <= (rel + ulp16) * T && !(b == 0.f && fminf(a, d) == 0.f);
   if (!bsmall) {
-    const float cond = (24.0f / pi) * ulp16 * T / fmaxf(2.0f * delta, 1e-30f);
+    const float cond = (24.0f / pi) * ulp16 * T * __builtin_amdgcn_rcpf(fmaxf(2.0f * delta, 1e-30f));
     n |= fabsf(u - rintf(u)) <= 24.0f * rel + cond;
   }
   if (RULE == RULE_LITE) {

@@ -25,6 +25,11 @@ struct NnediWeights {
   const void* w_lo = nullptr;
   const float* bias = nullptr;
   const int* wexp = nullptr;
+  // launch shape (set by the autotuner; 0 = defaults): persistent blocks per CU (capped at
+  // the kernel's occupancy), and for nns = 128 the neuron split (0: 2 slices x 4 tiles,
+  // 1: 4 slices x 2 tiles per wave)
+  int grid_per_cu = 0;
+  int split = 0;
 };
 
 int nnedi3_prepare(mpx_ctx* ctx, int nns, int wh, int precision, NnediWeights* out);

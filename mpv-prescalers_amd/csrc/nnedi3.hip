@@ -509,7 +509,8 @@ static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t s
     if (!per_cu &&                                                                                    \
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1)) \
       per_cu = 1;                                                                                     \
-    const int grid = min(total, cu_count() * per_cu);                                                 \
+    const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
+    const int grid = min(total, cu_count() * k);                                                      \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
                        inv_acc, tiles_x, tiles_y, total);                                             \
   }
@@ -518,7 +519,10 @@ static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t s
     case 1: MPX_NN(FM, TERMS, 1, 1, SC) break;      \
     case 2: MPX_NN(FM, TERMS, 1, 2, SC) break;      \
     case 4: MPX_NN(FM, TERMS, 1, 4, SC) break;      \
-    case 8: MPX_NN(FM, TERMS, 2, 4, SC) break;      \
+    case 8:                                         \
+      if (nw.split) MPX_NN(FM, TERMS, 4, 2, SC)     \
+      else MPX_NN(FM, TERMS, 2, 4, SC)              \
+      break;                                        \
     case 16: MPX_NN(FM, TERMS, 4, 4, SC) break;     \
     default: return hipErrorInvalidValue;           \
   }

@@ -109,8 +109,11 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   }
   float r4[4] = {res.x, res.y, res.z, res.w};
   if (AR) {
-    const float h[4] = {hi2.x / hi.x - 0.1f, hi2.y / hi.y - 0.1f, hi2.z / hi.z - 0.1f, hi2.w / hi.w - 0.1f};
-    const float l[4] = {1.1f - lo2.x / lo.x, 1.1f - lo2.y / lo.y, 1.1f - lo2.z / lo.z, 1.1f - lo2.w / lo.w};
+    // quotients through the hardware reciprocal (within 1 ulp of the division)
+    const float h[4] = {hi2.x * __builtin_amdgcn_rcpf(hi.x) - 0.1f, hi2.y * __builtin_amdgcn_rcpf(hi.y) - 0.1f,
+                        hi2.z * __builtin_amdgcn_rcpf(hi.z) - 0.1f, hi2.w * __builtin_amdgcn_rcpf(hi.w) - 0.1f};
+    const float l[4] = {1.1f - lo2.x * __builtin_amdgcn_rcpf(lo.x), 1.1f - lo2.y * __builtin_amdgcn_rcpf(lo.y),
+                        1.1f - lo2.z * __builtin_amdgcn_rcpf(lo.z), 1.1f - lo2.w * __builtin_amdgcn_rcpf(lo.w)};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const float cl = fminf(fmaxf(r4[k], l[k]), h[k]);

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   for (int c = 0; c < P; ++c) {
     float v = res[c];
     if (AR) {
-      const float h = hi2[c] / hi[c] - 0.1f, l = 1.1f - lo2[c] / lo[c];
+      const float h = hi2[c] * __builtin_amdgcn_rcpf(hi[c]) - 0.1f, l = 1.1f - lo2[c] * __builtin_amdgcn_rcpf(lo[c]);
       const float cl = fminf(fmaxf(v, l), h);
       v = v * (1.0f - 0.8f) + cl * 0.8f;  // mix(res, clamp(res, lo, hi), 0.8), no final clamp
     } else {

@@ -77,6 +77,25 @@ class Plan:
     def run(self, x, out=None, stream=None, buckets=False):
         """Upscale ``x`` (device tensor) into ``out`` (allocated if None).  Returns ``out``,
         or ``(out, bucket_rows)`` with ``buckets=True``."""
+        io, x, out, bk = self._io(x, out, buckets)
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        check(LIB.mpx_run_ex(self._h, ctypes.byref(io), ctypes.c_void_p(stream.cuda_stream)))
+        return (out, bk) if buckets else out
+
+    def tune(self, x, out=None, iters=5, stream=None):
+        """Autotune the launch shape on ``x`` / ``out`` (writes ``out``); the plan keeps the
+        fastest.  Returns {"grid_per_cu", "split", "us"} (grid 0 = occupancy maximum;
+        plans without a tunable shape return zeros)."""
+        io, x, out, _ = self._io(x, out, False)
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        g, sp, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        check(LIB.mpx_plan_tune(self._h, ctypes.byref(io), ctypes.c_void_p(stream.cuda_stream), iters,
+                                ctypes.byref(g), ctypes.byref(sp), ctypes.byref(us)))
+        return {"grid_per_cu": g.value, "split": sp.value, "us": us.value}
+
+    def _io(self, x, out, buckets):
         planes = self.info["planes"]
         if x.dtype != self.in_dtype:
             raise TypeError("plan expects %s input, got %s" % (self.in_dtype, x.dtype))
@@ -113,10 +132,7 @@ class Plan:
         io.out_bstride = out.stride(0) if out.dim() == nd + 1 else out.numel()
         io.batch = batch
         io.buckets = bk.data_ptr() if bk is not None else None
-        if stream is None:
-            stream = torch.cuda.current_stream(x.device)
-        check(LIB.mpx_run_ex(self._h, ctypes.byref(io), ctypes.c_void_p(stream.cuda_stream)))
-        return (out, bk) if buckets else out
+        return io, x, out, bk
 
 
     def run_frames(self, xs, outs=None, stream=None):

@@ -1,0 +1,34 @@
+"""Launch-shape autotuner: picks a candidate, keeps it in the plan, and the tuned plan
+still meets the parity bar."""
+import numpy as np
+import pytest
+import torch
+
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("nns,wh", [(32, 4), (128, 6)])
+def test_tune_nnedi3(nns, wh):
+    import mpx
+    from mpx.frames import make_plane
+    from oracle.nnedi3 import nnedi3
+    img = make_plane(160, 96, 7, bits=10).astype(np.float32)
+    x = torch.from_numpy(img).cuda()
+    plan = mpx.Context(0).plan("nnedi3-nns%d-win8x%d" % (nns, wh), 160, 96, precision="fp16")
+    res = plan.tune(x, iters=3)
+    assert res["grid_per_cu"] in (0, 1, 2, 3, 4, 6, 8) and res["us"] > 0
+    assert res["split"] in ((0, 1) if nns == 128 else (0,))
+    f = compare("nnedi3-nns%d-win8x%d fp16 tuned %s" % (nns, wh, res), plan.run(x).cpu().numpy(),
+                nnedi3(img.astype(np.float64), nns, wh))
+    assert f["max_abs"] <= 2.0 ** -10
+
+
+def test_tune_other_plans_untouched():
+    import mpx
+    x = torch.rand((32, 48), device="cuda")
+    plan = mpx.Context(0).plan("ravu-lite-r2", 48, 32)
+    before = plan.run(x).clone()
+    assert plan.tune(x) == {"grid_per_cu": 0, "split": 0, "us": 0.0}
+    assert torch.equal(plan.run(x), before)
