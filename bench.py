@@ -145,8 +145,28 @@ def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
     finally:
         srv.close()
     # (c) one frame per launch on one stream: the plain request loop (launch, wait for the
-    # result, next request) and, for reference, the same launches issued back to back
+    # result, next request), the same launches issued back to back, and the plain request
+    # loop with the plain forms (FP32 VALU NNEDI3)
+    single_sync = _one_per_launch(pool, reqs, prec, total_px, sync=True)
+    single_async = _one_per_launch(pool, reqs, prec, total_px, sync=False)
+    plain_sync = _one_per_launch(pool, reqs, {v: "fp32" for v in prec}, total_px, sync=True)
+    small = _small_frames(prec)
+    return {"workload": "Poisson %.0f frames/s, sizes %s, variants %s, 10-bit fp16 frames" %
+                        (rate, "/".join("%dx%d" % s_ for s_ in SERVE_SIZES), "/".join(SERVE_VARIANTS)),
+            "offered_load": offered, "saturated_mpx_s": round(sat, 1),
+            "one_frame_per_launch_sync_mpx_s": round(single_sync, 1),
+            "one_frame_per_launch_async_mpx_s": round(single_async, 1),
+            "one_frame_per_launch_plain_fp32_sync_mpx_s": round(plain_sync, 1),
+            "speedup_vs_one_frame_per_launch_sync": round(sat / single_sync, 2),
+            "speedup_vs_one_frame_per_launch_async": round(sat / single_async, 2),
+            "speedup_vs_one_frame_per_launch_plain_fp32_sync": round(sat / plain_sync, 2),
+            "launches": launches, "small_frames": small}
+
+
+def _one_per_launch(pool, reqs, prec, total_px, sync):
+    """Throughput (Mpixel/s) of one launch per request on one stream."""
     import mpx
+    import torch
     ctx = mpx.Context(0)
     plans = {}
     for key, frames in pool.items():
@@ -157,20 +177,46 @@ def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
     t0 = time.perf_counter()
     for v, x in reqs:
         plans[(v, x.shape[-1], x.shape[-2])].run(x)
-        torch.cuda.synchronize()
-    single_sync = total_px / (time.perf_counter() - t0) / 1e6
-    t0 = time.perf_counter()
-    for v, x in reqs:
-        plans[(v, x.shape[-1], x.shape[-2])].run(x)
+        if sync:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
-    single_async = total_px / (time.perf_counter() - t0) / 1e6
-    return {"workload": "Poisson %.0f frames/s, sizes %s, variants %s, 10-bit fp16 frames" %
-                        (rate, "/".join("%dx%d" % s_ for s_ in SERVE_SIZES), "/".join(SERVE_VARIANTS)),
-            "offered_load": offered, "saturated_mpx_s": round(sat, 1),
-            "one_frame_per_launch_sync_mpx_s": round(single_sync, 1),
-            "one_frame_per_launch_async_mpx_s": round(single_async, 1),
-            "speedup_vs_one_frame_per_launch_sync": round(sat / single_sync, 2),
-            "speedup_vs_one_frame_per_launch_async": round(sat / single_async, 2), "launches": launches}
+    mpx_s = total_px / (time.perf_counter() - t0) / 1e6
+    for p_ in plans.values():
+        p_.close()
+    ctx.close()
+    return mpx_s
+
+
+def _small_frames(prec, n=2000, w=640, h=360):
+    """All requests 640x360 (the same four variants, round robin), queued at once through
+    the loop vs one launch per request (synchronous): where launches dominate, batching
+    pays."""
+    import numpy as np
+    import torch
+    from mpx.frames import make_frame
+    from mpx.serving import Server
+    dev = torch.device("cuda:0")
+    pool = {(v, w, h): [torch.from_numpy(make_frame(w, h, 3 + i, bits=10, channels=3 if v.endswith("-yuv") else 1)
+                                         .astype(np.float16)).to(dev) for i in range(2)] for v in SERVE_VARIANTS}
+    reqs = [(SERVE_VARIANTS[i % len(SERVE_VARIANTS)], pool[(SERVE_VARIANTS[i % len(SERVE_VARIANTS)], w, h)][i % 2])
+            for i in range(n)]
+    total_px = sum((9 if v.startswith("ravu-3x") else 4) * w * h * (3 if v.endswith("-yuv") else 1) for v, _ in reqs)
+    srv = Server(device=0, hbm_budget=16 << 30, streams=4, max_batch=16)
+    try:
+        for key, frames in pool.items():
+            srv.submit(key[0], frames[0], precision=prec[key[0]]).wait(120)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hs = [srv.submit(v, x, precision=prec[v]) for v, x in reqs]
+        for r in hs:
+            r.wait(300)
+        sat = total_px / (max(r.t_done for r in hs) - t0) / 1e6
+    finally:
+        srv.close()
+    single = _one_per_launch(pool, reqs, prec, total_px, sync=True)
+    return {"workload": "%d requests of %dx%d, variants round robin" % (n, w, h), "saturated_mpx_s": round(sat, 1),
+            "one_frame_per_launch_sync_mpx_s": round(single, 1),
+            "speedup_vs_one_frame_per_launch_sync": round(sat / single, 2)}
 
 
 def _double_x(ctx, torch):
@@ -182,6 +228,7 @@ def _double_x(ctx, torch):
     plan = ctx.plan(HEADLINE, W, 2 * H, target=(2 * W, 2 * H), in_dtype=torch.float32,
                     out_dtype=torch.float16, precision="fp16")
     assert plan.info["scale_x"] == 2 and plan.info["scale_y"] == 1
+    plan.tune(xi, xo, iters=5)
     return lambda: plan.run(xi, out=xo)
 
 
@@ -207,8 +254,9 @@ def dominant_kernel(ctx, steps, torch):
 
 
 def kernel_only(ctx, steps, torch):
-    """The dominant kernel `steps` times, then a plain device copy of 256 MiB `steps`
-    times (the calibration for the FETCH_SIZE / WRITE_SIZE counters)."""
+    """The dominant kernel autotuned (7 candidate shapes x 6 runs), then `steps` times,
+    then a plain device copy of 256 MiB `steps` times (the calibration for the
+    FETCH_SIZE / WRITE_SIZE counters)."""
     fn = _double_x(ctx, torch)
     for _ in range(steps):
         fn()
@@ -256,6 +304,7 @@ def main():
 
     plans = {p: ctx.plan(HEADLINE, W, H, in_dtype=torch.float16, out_dtype=torch.float16, precision=p)
              for p in ("fp16", "fp32")}
+    launch = plans["fp16"].tune(x, out, iters=5)   # autotuned launch shape, kept by the plan
     res = {}
     for p, plan in plans.items():
         n = steps if p == "fp16" else max(10, steps // 10)
@@ -275,6 +324,7 @@ def main():
             pl = ctx.plan("nnedi3-nns256-win8x6", W, H, in_dtype=torch.float16, out_dtype=torch.float16,
                           precision=p, max_batch=b)
             n = 5 if p == "fp16" else 2
+            pl.tune(f10, o10, iters=2)
             t, wall = timed(lambda: pl.run(f10, out=o10), n, 1, torch)
             fps = b * n / wall
             secondary["nnedi3-nns256-win8x6 %s b8" % p] = {
@@ -318,7 +368,8 @@ def main():
         "dtype": "fp16-mfma/fp32-acc",
         "data": "synthetic (seeded gradient + edges + zone plate + noise, 8-bit quantised)",
         "config": {"workload": "nnedi3-nns32-win8x4 luma 2x, 1920x1080 fp16 plane -> 3840x2160, batch 1",
-                   "variant": HEADLINE, "precision": "fp16", "in": [W, H], "out": [2 * W, 2 * H]},
+                   "variant": HEADLINE, "precision": "fp16", "in": [W, H], "out": [2 * W, 2 * H],
+                   "launch": {"grid_per_cu": launch["grid_per_cu"], "split": launch["split"]}},
         "plain_fp32": {"value": round(res["fp32"]["mpx_s"], 1), "unit": "Mpixel/s",
                        "ms_per_step": round(res["fp32"]["ms_per_step"], 4)},
         "speedup_vs_plain_fp32": round(f16["mpx_s"] / res["fp32"]["mpx_s"], 2),
