@@ -27,6 +27,14 @@ PEAK_FP16 = 2500.0  # TFLOP/s, dense FP16 MFMA (MI355X)
 PEAK_HBM = 8000.0   # GB/s
 
 
+# (variant, input size, target, planes, batch, dtype name) of the RAVU secondary lines
+RAVU_SECONDARY = (("ravu-lite-ar-r3", (640, 360), None, 1, 64, "float32"),
+                  ("ravu-lite-ar-r3", (1920, 1080), None, 1, 8, "float16"),
+                  ("ravu-zoom-ar-r2-rgb", (1280, 720), (2560, 1440), 3, 4, "float16"),
+                  ("ravu-r4-yuv", (1920, 1080), None, 3, 2, "float16"),
+                  ("ravu-3x-r3", (1920, 1080), None, 1, 2, "float16"))
+
+
 def nnedi3_flop_per_px(name):
     nns = int(name.split("nns")[1].split("-")[0])
     k = 8 * int(name.split("win8x")[1])
@@ -269,6 +277,42 @@ def kernel_only(ctx, steps, torch):
     return 0
 
 
+def kernel_only_nns256(ctx, steps, torch):
+    """nnedi3-nns256-win8x6 FP16 MFMA on 8 10-bit 1080p frames: autotuned, then `steps`
+    launches (both passes)."""
+    import numpy as np
+    from mpx.frames import make_plane
+    b = 8
+    x = torch.from_numpy(np.stack([make_plane(W, H, seed=20 + i, bits=10) for i in range(b)])
+                         .astype(np.float16)).to("cuda:0")
+    o = torch.empty((b, 2 * H, 2 * W), dtype=torch.float16, device="cuda:0")
+    pl = ctx.plan("nnedi3-nns256-win8x6", W, H, in_dtype=torch.float16, out_dtype=torch.float16,
+                  precision="fp16", max_batch=b)
+    r = pl.tune(x, o, iters=2)
+    for _ in range(steps):
+        pl.run(x, out=o)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "nnedi3-nns256-win8x6 fp16 b8 1080p", "launch": r, "launches": steps}), flush=True)
+    return 0
+
+
+def kernel_only_ravu(ctx, steps, torch):
+    """The RAVU variants of the secondary list, `steps` launches each."""
+    import numpy as np
+    from mpx.frames import make_plane
+    for name, (iw, ih), tgt, planes, b, dt in RAVU_SECONDARY:
+        dt = getattr(torch, dt)
+        fr = np.stack([make_plane(iw, ih, seed=i) for i in range(b * planes)]).astype(np.float32)
+        xin = torch.from_numpy(fr.reshape((b, planes, ih, iw) if planes == 3 else (b, ih, iw))).to(dt).to("cuda:0")
+        pl = ctx.plan(name, iw, ih, target=tgt, in_dtype=dt, out_dtype=dt, max_batch=b)
+        out = pl.run(xin)
+        for _ in range(steps):
+            pl.run(xin, out=out)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": [r[0] for r in RAVU_SECONDARY], "launches_each": steps}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +324,10 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant kernel (double_x pass) and a plain copy of known size, for "
                          "hardware-counter runs")
+    ap.add_argument("--workload", default="headline", choices=("headline", "nns256", "ravu"),
+                    help="with --kernel-only: the headline double_x pass (+ copy calibration), "
+                         "nnedi3-nns256-win8x6 on 8 10-bit 1080p frames, or the RAVU variants of "
+                         "the secondary list")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (frames shard trivially across "
@@ -300,6 +348,10 @@ def main():
     steps, warm = args.steps, args.warmup
 
     if args.kernel_only:
+        if args.workload == "nns256":
+            return kernel_only_nns256(ctx, args.steps, torch)
+        if args.workload == "ravu":
+            return kernel_only_ravu(ctx, args.steps, torch)
         return kernel_only(ctx, args.steps, torch)
 
     plans = {p: ctx.plan(HEADLINE, W, H, in_dtype=torch.float16, out_dtype=torch.float16, precision=p)
@@ -330,11 +382,8 @@ def main():
             secondary["nnedi3-nns256-win8x6 %s b8" % p] = {
                 "fps": round(fps, 1), "mpx_s": round(fps * 4 * W * H / 1e6, 1),
                 "tflops": round(fps * nnedi3_flop_per_px("nnedi3-nns256-win8x6") * 3 * W * H / 1e12, 1)}
-        for name, (iw, ih), tgt, planes, b, dt in (
-                ("ravu-lite-ar-r3", (640, 360), None, 1, 64, torch.float32),
-                ("ravu-zoom-ar-r2-rgb", (1280, 720), (2560, 1440), 3, 4, torch.float16),
-                ("ravu-r4-yuv", (1920, 1080), None, 3, 2, torch.float16),
-                ("ravu-3x-r3", (1920, 1080), None, 1, 2, torch.float16)):
+        for name, (iw, ih), tgt, planes, b, dt in RAVU_SECONDARY:
+            dt = getattr(torch, dt)
             fr = np.stack([make_plane(iw, ih, seed=i) for i in range(b * planes)]).astype(np.float32)
             xin = torch.from_numpy(fr.reshape((b, planes, ih, iw) if planes == 3 else (b, ih, iw))).to(dt).to(dev)
             pl = ctx.plan(name, iw, ih, target=tgt, in_dtype=dt, out_dtype=dt, max_batch=b)
@@ -343,7 +392,8 @@ def main():
             io_bytes = (iw * ih + opx) * planes * (4 if dt == torch.float32 else 2)
             secondary["%s %dx%d b%d" % (name, iw, ih, b)] = {
                 "mpx_s": round(b * 10 * opx / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
-                "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1)}
+                "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
+                "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4)}
 
     serving = None
     if not args.no_serving:

@@ -171,8 +171,9 @@ __device__ __forceinline__ T deriv(Line s, int p) {
   // multiplies by the reciprocals: exact for 1/2, within one rounding of the quotient for
   // 1/12.  FP64 (the re-decision) divides, as the oracle does.
   if (FIVE && p >= 2 && p + 2 <= N - 1) {
+    if constexpr (sizeof(T) == 4)  // same value, fused: (s[p-2] - s[p+2]) + 8 (s[p+1] - s[p-1])
+      return fmaf(8.0f, s(p + 1) - s(p - 1), s(p - 2) - s(p + 2)) * (1.0f / 12.0f);
     const T num = -s(p + 2) + T(8) * s(p + 1) - T(8) * s(p - 1) + s(p - 2);
-    if constexpr (sizeof(T) == 4) return num * T(1.0 / 12.0);
     return num / T(12);
   }
   if constexpr (sizeof(T) == 4) return (s(p + 1) - s(p - 1)) * T(0.5);
@@ -180,7 +181,9 @@ __device__ __forceinline__ T deriv(Line s, int p) {
 }
 
 // abd = sum over the gradient grid of (gx*gx, gx*gy, gy*gy) * G, grid X outer, Y inner
-// (ravu-lite-r3.hook:48-76).  get(i) returns window sample i = X*N + Y.
+// (ravu-lite-r3.hook:48-76).  get(i) returns window sample i = X*N + Y.  FP32 (the fast
+// path, trusted only away from bucket boundaries) fuses multiply-adds; FP64 (the
+// re-decision) keeps every rounding step of the formula, like the oracle.
 template <int N, bool FIVE, typename T, typename Get>
 __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
 #pragma clang fp contract(off)
@@ -195,9 +198,15 @@ __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
       T gx = deriv<N, FIVE, T>([&](int k) { return get(k * N + Y); }, X);
       T gy = deriv<N, FIVE, T>([&](int k) { return get(X * N + k); }, Y);
       const T w = T(gauss_w<N>(gi));
-      a = a + gx * gx * w;
-      b = b + gx * gy * w;
-      d = d + gy * gy * w;
+      if constexpr (sizeof(T) == 4) {  // fast path: fused accumulation
+        a = fmaf(gx * gx, w, a);
+        b = fmaf(gx * gy, w, b);
+        d = fmaf(gy * gy, w, d);
+      } else {                         // re-decision: the reference's rounding steps
+        a = a + gx * gx * w;
+        b = b + gx * gy * w;
+        d = d + gy * gy * w;
+      }
     }
   }
 }

@@ -18,7 +18,10 @@ import time
 
 import torch
 
+from ._abi import MpxError
 from .ops import Context, default_target
+
+_BUDGET = -5  # MPX_ERR_BUDGET
 
 
 class Request:
@@ -68,7 +71,8 @@ class Server:
         self.max_batch = max(1, min(16, max_batch))
         self.plan_batch = plan_batch or self.max_batch
         self.admit_bytes = admit_bytes if admit_bytes is not None else hbm_budget // 2
-        self._plans = {}
+        self._plans = collections.OrderedDict()   # (plan key, stream) -> Plan, LRU order
+        self._last_use = {}                        # (plan key, stream) -> event of its last launch
         self._queue = collections.deque()
         self._inflight = 0
         self._cv = threading.Condition()
@@ -121,43 +125,75 @@ class Server:
 
     # ------------------------------------------------------------------ scheduler
     def _plan(self, key, si):
+        """The plan of (key, stream), created on first use.  When the context's HBM budget
+        is exhausted, idle plans (their last launch finished) are evicted, oldest first."""
         p = self._plans.get((key, si))
-        if p is None:
-            name, in_w, in_h, tgt, in_dt, out_dt, prec, bits = key
-            p = self.ctx.plan(name, in_w, in_h, tgt, in_dt, out_dt, prec, self.plan_batch, bits)
-            self._plans[(key, si)] = p
+        if p is not None:
+            self._plans.move_to_end((key, si))
+            return p
+        name, in_w, in_h, tgt, in_dt, out_dt, prec, bits = key
+        while True:
+            try:
+                p = self.ctx.plan(name, in_w, in_h, tgt, in_dt, out_dt, prec, self.plan_batch, bits)
+                break
+            except MpxError as e:
+                if e.code != _BUDGET or not self._evict_one():
+                    raise
+        self._plans[(key, si)] = p
         return p
+
+    def _evict_one(self):
+        """Free the least recently used idle plan; if every plan has work in flight, wait
+        for the least recently used one to finish.  False when there is no plan."""
+        if not self._plans:
+            return False
+        victim = next((k for k in self._plans if self._last_use.get(k) is None or self._last_use[k].query()),
+                      next(iter(self._plans)))
+        ev = self._last_use.pop(victim, None)
+        if ev is not None:
+            ev.synchronize()
+        self._plans.pop(victim).close()
+        return True
+
+    def _admit_locked(self):
+        """Admit queued requests in arrival order while the outputs in flight fit
+        admit_bytes; requests whose plan cannot be made fail here."""
+        admitted = []
+        while self._queue:
+            r = self._queue[0]
+            if r.key is None:
+                in_h, in_w = r.frame.shape[-2:]
+                key = (r.name, in_w, in_h, r.target, r.frame.dtype, r.out_dtype, r.precision, r.bits)
+                try:
+                    info = self._plan(key, 0).info
+                except Exception as e:  # unknown / unavailable variant, bad size, budget
+                    self._queue.popleft()
+                    r.error = e
+                    r.t_done = time.perf_counter()
+                    r._done.set()
+                    continue
+                r.key = key
+                r.bytes = info["planes"] * info["out_w"] * info["out_h"] * torch.finfo(r.out_dtype).bits // 8
+            if self._inflight + r.bytes > self.admit_bytes and self._inflight > 0:
+                break
+            self._queue.popleft()
+            self._inflight += r.bytes
+            r.t_admit = time.perf_counter()
+            admitted.append(r)
+        return admitted
 
     def _schedule(self):
         while True:
             with self._cv:
-                while (not self._queue or self._paused) and not self._stop:
-                    self._cv.wait()
-                if self._stop and not self._queue:
-                    return
-                admitted = []
-                # admit in arrival order while the output bytes fit the budget
-                while self._queue:
-                    r = self._queue[0]
-                    if r.key is None:
-                        in_h, in_w = r.frame.shape[-2:]
-                        key = (r.name, in_w, in_h, r.target, r.frame.dtype, r.out_dtype, r.precision, r.bits)
-                        try:
-                            info = self._plan(key, 0).info
-                        except Exception as e:  # unknown / unavailable variant, bad size, budget
-                            self._queue.popleft()
-                            r.error = e
-                            r.t_done = time.perf_counter()
-                            r._done.set()
-                            continue
-                        r.key = key
-                        r.bytes = info["planes"] * info["out_w"] * info["out_h"] * torch.finfo(r.out_dtype).bits // 8
-                    if self._inflight + r.bytes > self.admit_bytes and self._inflight > 0:
+                while True:
+                    if self._stop and not self._queue:
+                        return
+                    admitted = self._admit_locked() if (self._queue and not self._paused) else []
+                    if admitted:
                         break
-                    self._queue.popleft()
-                    self._inflight += r.bytes
-                    r.t_admit = time.perf_counter()
-                    admitted.append(r)
+                    # nothing admissible (empty queue, paused, or the outputs in flight fill
+                    # admit_bytes): sleep until a submit, resume or completion notifies
+                    self._cv.wait()
             groups = collections.OrderedDict()
             for r in admitted:
                 groups.setdefault(r.key, []).append(r)
@@ -178,6 +214,7 @@ class Server:
                 plan.run_frames([r.frame for r in reqs], outs, stream=st)
                 ev = torch.cuda.Event()
                 ev.record(st)
+            self._last_use[(key, si)] = ev
             for r, o in zip(reqs, outs):
                 r.out = o
             self.launches += 1

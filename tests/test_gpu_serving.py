@@ -87,3 +87,20 @@ def test_bad_request_fails_alone():
         assert good.wait(timeout=30).shape == (64, 64)
     finally:
         srv.close()
+
+
+def test_plan_eviction_under_context_budget():
+    """A context budget that holds one plan's scratch at a time: plans of other sizes are
+    made by evicting idle ones, and every request completes with the result it has alone."""
+    from mpx.serving import Server
+    import mpx
+    sizes = [(256, 256), (200, 200), (160, 160), (256, 256)]
+    budget = 64 * 1024 + 256 * 256 * 4 + 32 * 1024   # weights + the largest int11 scratch + slack
+    srv = Server(device=0, hbm_budget=budget, admit_bytes=1 << 30, streams=1, max_batch=1)
+    try:
+        frames = [_frame(w, h, 1, 30 + i) for i, (w, h) in enumerate(sizes)]
+        rs = [srv.submit("ravu-r2", f) for f in frames]
+        for f, r in zip(frames, rs):
+            assert torch.equal(r.wait(timeout=60), mpx.upscale("ravu-r2", f))
+    finally:
+        srv.close()
