@@ -136,6 +136,9 @@ std::string default_weights_dir() {
 
 int elem_size(int t) { return t == MPX_F32 ? 4 : (t == MPX_U8 ? 1 : 2); }
 
+// Kernels index tiles and blocks of one launch with 32-bit integers.
+constexpr int64_t kMaxLaunchPixels = (int64_t{1} << 31) - 1;
+
 }  // namespace
 
 // ----------------------------------------------------------------------------- context
@@ -164,7 +167,6 @@ int mpx_create(int device, uint64_t hbm_budget, const char* weights_dir, mpx_ctx
   int n = 0;
   HIP_TRY(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(MPX_ERR_ARG, "device " + std::to_string(device) + " out of range");
-  HIP_TRY(hipSetDevice(device));
   auto* c = new mpx_ctx();
   c->device = device;
   c->budget = hbm_budget;
@@ -203,7 +205,9 @@ int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int targ
     return fail(MPX_ERR_SHAPE, "sizes out of range");
   if (in_type < MPX_U8 || in_type > MPX_F32) return fail(MPX_ERR_ARG, "bad in_type");
   if (out_type != MPX_F16 && out_type != MPX_F32) return fail(MPX_ERR_ARG, "out_type must be MPX_F16 or MPX_F32");
-  if (max_batch < 1 || max_batch > 65535) return fail(MPX_ERR_SHAPE, "max_batch out of range");
+  // the pass-through copy launches one grid layer per frame and plane (at most 65535)
+  if (max_batch < 1 || max_batch * (v->channels == MPX_CH_LUMA ? 1 : 3) > 65535)
+    return fail(MPX_ERR_SHAPE, "max_batch out of range");
   if (in_type == MPX_U16 && (bits < 1 || bits > 16)) return fail(MPX_ERR_ARG, "bits must be 1..16 for MPX_U16");
   if (precision < MPX_PREC_FP32 || precision > MPX_PREC_FP16X3) return fail(MPX_ERR_ARG, "bad precision");
   if (v->family != MPX_FAMILY_NNEDI3 && precision != MPX_PREC_FP32)
@@ -298,7 +302,10 @@ int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
                         io->out_bstride < io->out_pitch * inf.out_h * inf.planes))
     return fail(MPX_ERR_ARG, "frame stride smaller than a frame");
   if (io->buckets && !inf.applies) return fail(MPX_ERR_ARG, "buckets requested from a skipped plan");
-  HIP_TRY(hipSetDevice(plan->ctx->device));
+  if (static_cast<int64_t>(inf.out_w) * inf.out_h * io->batch > kMaxLaunchPixels)
+    return fail(MPX_ERR_SHAPE, "more than 2^31 output pixels per plane in one launch");
+  mpx_host::DeviceScope dscope;
+  if (int drc = dscope.enter(plan->ctx->device)) return drc;
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
   mpx::Io d{};
   d.ntab = 0;
@@ -331,7 +338,10 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
   if (in_pitch < plan->in_w || out_pitch < inf.out_w) return fail(MPX_ERR_ARG, "pitch smaller than the row width");
   for (int i = 0; i < n; ++i)
     if (!ins[i] || !outs[i]) return fail(MPX_ERR_ARG, "frame " + std::to_string(i) + " has a NULL plane");
-  HIP_TRY(hipSetDevice(plan->ctx->device));
+  if (static_cast<int64_t>(inf.out_w) * inf.out_h * n > kMaxLaunchPixels)
+    return fail(MPX_ERR_SHAPE, "more than 2^31 output pixels per plane in one launch");
+  mpx_host::DeviceScope dscope;
+  if (int drc = dscope.enter(plan->ctx->device)) return drc;
   mpx::Io d{};
   d.ntab = n;
   for (int i = 0; i < n; ++i) {
@@ -419,7 +429,7 @@ int mpx_resample(const mpx_io* io, int in_w, int in_h, int in_type, int bits, in
     return fail(MPX_ERR_SHAPE, "sizes outside 1..32768");
   if (!std::isfinite(offset_x) || !std::isfinite(offset_y) || std::fabs(offset_x) > 64.f || std::fabs(offset_y) > 64.f)
     return fail(MPX_ERR_ARG, "offset outside [-64, 64]");
-  if (io->batch < 1 || io->batch > 65535) return fail(MPX_ERR_SHAPE, "batch outside 1..65535");
+  if (io->batch < 1 || io->batch * planes > 65535) return fail(MPX_ERR_SHAPE, "batch * planes outside 1..65535");
   if (io->in_pitch < in_w || io->out_pitch < out_w) return fail(MPX_ERR_ARG, "pitch smaller than the row width");
   if (planes > 1 && (io->in_cstride < io->in_pitch * in_h || io->out_cstride < io->out_pitch * out_h))
     return fail(MPX_ERR_ARG, "plane stride smaller than a plane");
@@ -455,7 +465,8 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
   if (best_us) *best_us = 0.f;
   const bool tunable = plan->v->family == MPX_FAMILY_NNEDI3 && plan->precision != MPX_PREC_FP32 && plan->info.applies;
   if (!tunable) return MPX_OK;
-  HIP_TRY(hipSetDevice(plan->ctx->device));
+  mpx_host::DeviceScope dscope;
+  if (int drc = dscope.enter(plan->ctx->device)) return drc;
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));

@@ -35,6 +35,29 @@ inline int fail(int code, const std::string& msg) {
       return ::mpx_host::fail(MPX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Makes a device current for a scope and gives the calling thread its previous device
+// back at the end (entry points must not change the caller's current device).
+class DeviceScope {
+ public:
+  DeviceScope() = default;
+  DeviceScope(const DeviceScope&) = delete;
+  DeviceScope& operator=(const DeviceScope&) = delete;
+  int enter(int dev) {
+    if (hipGetDevice(&prev_) != hipSuccess) prev_ = -1;
+    if (prev_ == dev) return MPX_OK;
+    HIP_TRY(hipSetDevice(dev));
+    set_ = true;
+    return MPX_OK;
+  }
+  ~DeviceScope() {
+    if (set_ && prev_ >= 0) (void)hipSetDevice(prev_);
+  }
+
+ private:
+  int prev_ = -1;
+  bool set_ = false;
+};
+
 inline bool read_weight_file(const std::string& path, std::vector<float>& data, std::vector<uint32_t>& dims) {
   FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) return false;
@@ -75,22 +98,30 @@ struct mpx_ctx {
   uint64_t budget = 0;
   uint64_t used = 0;
   std::string wdir;
-  std::mutex mu;
+  std::mutex mu;    // weight tables
+  std::mutex amu;   // budget accounting (taken inside mu, never the other way round)
   std::map<std::string, DeviceBuf> weights;            // resident weight tables
   std::map<std::string, std::vector<uint32_t>> dims;   // their host-side shapes
 
+  // Device memory under the budget; thread-safe (plans may be made and destroyed from
+  // several threads).
   int alloc(size_t bytes, void** out) {
+    std::lock_guard<std::mutex> lk(amu);
     if (budget && used + bytes > budget)
       return fail(MPX_ERR_BUDGET, "allocation of " + std::to_string(bytes) + " B exceeds the HBM budget (" +
                                       std::to_string(used) + " of " + std::to_string(budget) + " B used)");
-    HIP_TRY(hipSetDevice(device));
+    mpx_host::DeviceScope ds;
+    int rc = ds.enter(device);
+    if (rc) return rc;
     HIP_TRY(hipMalloc(out, bytes));
     used += bytes;
     return MPX_OK;
   }
   void release(void* p, size_t bytes) {
     if (!p) return;
-    (void)hipSetDevice(device);
+    std::lock_guard<std::mutex> lk(amu);
+    mpx_host::DeviceScope ds;
+    (void)ds.enter(device);
     (void)hipFree(p);
     used -= bytes;
   }
