@@ -463,8 +463,8 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
   if (grid_per_cu) *grid_per_cu = 0;
   if (split) *split = 0;
   if (best_us) *best_us = 0.f;
-  const bool tunable = plan->v->family == MPX_FAMILY_NNEDI3 && plan->precision != MPX_PREC_FP32 && plan->info.applies;
-  if (!tunable) return MPX_OK;
+  const bool nn = plan->v->family == MPX_FAMILY_NNEDI3 && plan->precision != MPX_PREC_FP32;
+  if (!nn || !plan->info.applies) return MPX_OK;
   mpx_host::DeviceScope dscope;
   if (int drc = dscope.enter(plan->ctx->device)) return drc;
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
@@ -475,14 +475,17 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
     return fail(MPX_ERR_HIP, "hipEventCreate failed");
   }
   const int grids[] = {0, 1, 2, 3, 4, 6, 8};
-  const int nsplit = plan->v->nns == 128 ? 2 : 1;
+  const int nsplit = nn && plan->v->nns == 128 ? 2 : 1;
   const mpx::NnediWeights saved = plan->nw;
+  auto set_shape = [&](int g, int sp) {
+    plan->nw.grid_per_cu = g;
+    plan->nw.split = sp;
+  };
   float best = 0.f;
   int best_g = 0, best_s = 0, rc = MPX_OK;
   for (int sp = 0; sp < nsplit && rc == MPX_OK; ++sp)
     for (int g : grids) {
-      plan->nw.grid_per_cu = g;
-      plan->nw.split = sp;
+      set_shape(g, sp);
       rc = mpx_run_ex(plan, io, hip_stream);  // warm-up; also validates io
       if (rc != MPX_OK) break;
       if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(MPX_ERR_HIP, "hipEventRecord failed"); break; }
@@ -507,8 +510,7 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
     plan->nw = saved;
     return rc;
   }
-  plan->nw.grid_per_cu = best_g;
-  plan->nw.split = best_s;
+  set_shape(best_g, best_s);
   if (grid_per_cu) *grid_per_cu = best_g;
   if (split) *split = best_s;
   if (best_us) *best_us = best;

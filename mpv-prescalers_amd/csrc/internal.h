@@ -184,7 +184,7 @@ struct mpx_plan {
   float in_max = 1.f;
   const __half* lut = nullptr;        // main LUT (RAVU family)
   const __half* lut_ar = nullptr;     // zoom-ar LUT
-  mpx::NnediWeights nw{};             // NNEDI3 packed weights
+  mpx::NnediWeights nw{};             // NNEDI3 packed weights (and their launch shape)
   mpx::ZoomPlan zoom{};               // zoom phase tables
   void* scratch = nullptr;            // intermediates, max_batch frames
   size_t scratch_bytes = 0;

@@ -8,6 +8,20 @@
 
 namespace mpx {
 
+// Compute units of the current device (queried once; 256 on MI355X).
+inline int device_cu_count() {
+  static int n = 0;
+  if (!n) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      n = v;
+    else
+      n = 256;
+  }
+  return n;
+}
+
 // Plane copy with element-type conversion (the output of a hook whose WHEN is false).
 hipError_t launch_copy(const Io& io, int planes, hipStream_t st);
 

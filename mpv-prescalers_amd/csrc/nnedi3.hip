@@ -477,17 +477,6 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
 
 // ----------------------------------------------------------------------------- launcher
 
-static int cu_count() {
-  static int n = 0;
-  if (!n) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-      n = 256;
-  }
-  return n;
-}
-
 template <int WH, int AXIS>
 static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t st) {
   const int tiles_x = (p.W + NX - 1) / NX, tiles_y = (p.H + NY - 1) / NY;
@@ -510,7 +499,7 @@ static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t s
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1)) \
       per_cu = 1;                                                                                     \
     const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
-    const int grid = min(total, cu_count() * k);                                                      \
+    const int grid = min(total, device_cu_count() * k);                                                      \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
                        inv_acc, tiles_x, tiles_y, total);                                             \
   }

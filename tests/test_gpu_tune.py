@@ -28,7 +28,7 @@ def test_tune_nnedi3(nns, wh):
 def test_tune_other_plans_untouched():
     import mpx
     x = torch.rand((32, 48), device="cuda")
-    plan = mpx.Context(0).plan("ravu-lite-r2", 48, 32)
+    plan = mpx.Context(0).plan("ravu-3x-r2", 48, 32)
     before = plan.run(x).clone()
     assert plan.tune(x) == {"grid_per_cu": 0, "split": 0, "us": 0.0}
     assert torch.equal(plan.run(x), before)
