@@ -254,14 +254,23 @@ def dominant_kernel(ctx, steps, torch):
     flop = nnedi3_flop_per_px(HEADLINE) * W * 2 * H
     achieved = flop / (kms * 1e-3) / 1e12
     traffic = None
+    valu_share = None
     tf = os.path.join(ROOT, "profiles", "traffic_nnedi3_double_x_nns32_1920x2160.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            rec = json.load(f)
+        traffic = rec.get("hbm_bytes_per_launch")
+        if rec.get("valu_instructions_per_launch"):
+            # issue model: 4 cycles per VALU instruction, 16 per transcendental, over the
+            # 1024 SIMDs at 2.4 GHz, as a share of the live kernel time
+            v, t = rec["valu_instructions_per_launch"], rec["valu_transcendental_instructions_per_launch"]
+            valu_share = round(((v - t) * 4 + t * 16) / 1024 / 2.4e9 / (kms * 1e-3), 3)
     return {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_FP16, "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP16, 4), "traffic": traffic,
             "kernel": "nnedi3_mfma double_x (fp16, nns32, win8x4)", "kernel_ms": round(kms, 5),
-            "flop_per_launch": flop, "algorithmic_io_bytes": W * 2 * H * 4 + 2 * W * 2 * H * 2}
+            "flop_per_launch": flop, "algorithmic_io_bytes": W * 2 * H * 4 + 2 * W * 2 * H * 2,
+            "binding_resource": "VALU issue (FP32 normalisation + exp2/rcp epilogue)",
+            "valu_issue_share": valu_share}
 
 
 def kernel_only(ctx, steps, torch):
