@@ -79,15 +79,16 @@ __device__ __forceinline__ void normalise(float* x, float& m, float& sd) {
   float sum = 0.f;
 #pragma unroll
   for (int k = 0; k < K; ++k) sum += x[k];
-  m = sum / static_cast<float>(K);
+  m = sum * (1.0f / static_cast<float>(K));
   float ss = 0.f;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     x[k] -= m;
     ss += x[k] * x[k];
   }
-  const float v = ss / static_cast<float>(K);
-  const float is = v >= kVarEps ? 1.0f / sqrtf(v) : 0.f;
+  const float v = ss * (1.0f / static_cast<float>(K));
+  // hardware reciprocal square root (within 1 ulp); the threshold test is exact
+  const float is = v >= kVarEps ? __builtin_amdgcn_rsqf(v) : 0.f;
   sd = v * is;
 #pragma unroll
   for (int k = 0; k < K; ++k) x[k] *= is;
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
     wsum += e;
     vsum = fmaf(e, q, vsum);
   }
-  const float pred = fminf(fmaxf(m + 5.0f * vsum / wsum * sd, 0.f), 1.f);
+  const float pred = fminf(fmaxf(m + 5.0f * vsum * __builtin_amdgcn_rcpf(wsum) * sd, 0.f), 1.f);
   store_result<AXIS>(p, frame_dst(p, f), x, y, tile[G::centre(tx, ty)], pred);
 }
 
@@ -468,7 +469,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
 
     const int x = x0 + tx, y = y0 + ty;
     if (x < p.W && y < p.H) {
-      const float pred = fminf(fmaxf(m + 5.0f * vsum / wsum * sd, 0.f), 1.f);
+      const float pred = fminf(fmaxf(m + 5.0f * vsum * __builtin_amdgcn_rcpf(wsum) * sd, 0.f), 1.f);
       store_result<AXIS>(p, frame_dst(p, f), x, y, orig, pred);
     }
     if constexpr (NSL == 1) wave_lds_sync();  // this tile's row reads before the next tile's writes
