@@ -82,6 +82,10 @@ __device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, f
 // leaves the sample's own page), its bit offset kept beside it.  `commit` extracts and
 // converts the samples and writes LDS.  A tile therefore costs one memory latency, and a
 // persistent block can issue its next tile while it computes this one.
+// A pointer into global memory (address space 1): loads through it are global loads, never
+// flat ones (which would also count against the LDS wait counter).
+typedef const __attribute__((address_space(1))) uint32_t* GlobalU32;
+
 template <int P, int PH, int PW, int NT>
 struct HaloTile {
   static constexpr int PN = PH * PW;
@@ -94,6 +98,22 @@ struct HaloTile {
                                         int ox, int oy) {
     const int lsh = dt == DT_F32 ? 2 : (dt == DT_U8 ? 0 : 1);  // log2 element bytes
     const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+    if (ox >= 0 && oy >= 0 && ox + PW <= W && oy + PH <= H) {
+      // interior tile (uniform branch): no clamping, and the per-thread part of each
+      // address (c*cstride + r*pitch + col) does not depend on the tile
+      const uintptr_t t0 = b + (static_cast<uintptr_t>(static_cast<int64_t>(oy) * pitch + ox) << lsh);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int i = min(static_cast<int>(threadIdx.x) + e * NT, N - 1);
+        const int c = P == 1 ? 0 : i / PN;
+        const int j = i - c * PN;
+        const int r = j / PW, col = j - r * PW;
+        const uintptr_t a = t0 + (static_cast<uintptr_t>(c * cstride + r * pitch + col) << lsh);
+        v[e] = *reinterpret_cast<GlobalU32>(a & ~static_cast<uintptr_t>(3));
+        sh[e] = static_cast<uint32_t>(a & 3) * 8;
+      }
+      return;
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = min(static_cast<int>(threadIdx.x) + e * NT, N - 1);  // tail lanes repeat the last sample
@@ -103,7 +123,7 @@ struct HaloTile {
       const int gx = min(max(ox + col, 0), W - 1);
       const int gy = min(max(oy + r, 0), H - 1);
       const uintptr_t a = b + (static_cast<uintptr_t>(c * cstride + static_cast<int64_t>(gy) * pitch + gx) << lsh);
-      v[e] = *reinterpret_cast<const uint32_t*>(a & ~static_cast<uintptr_t>(3));
+      v[e] = *reinterpret_cast<GlobalU32>(a & ~static_cast<uintptr_t>(3));
       sh[e] = static_cast<uint32_t>(a & 3) * 8;
     }
   }
