@@ -145,6 +145,10 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
   for (int c = 0; c < P; ++c) res[c] = 0.f;
 #pragma unroll
   for (int t = 0; t < RavuLut<N>::NTEX; ++t) {
+    // large windows (e.g. r4 with three planes: 192 samples): one LUT texel's taps at a
+    // time, so the loads of all samples are not hoisted together (register pressure,
+    // hence occupancy: measured -7 % time on ravu-r4-yuv; small windows lose by it)
+    if constexpr (N * N * P >= 96) __builtin_amdgcn_sched_barrier(0);
     const float4 w = load_lut4(lrow, t);
     const float a[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
