@@ -82,3 +82,34 @@ def test_errors():
     assert e.value.code == -2
     with pytest.raises(MpxError):
         mpx.query("ravu-lite-r3", 0, 10, 20, 20)
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    """include/mpx.h is plain C: a C program compiled with gcc against it links to
+    libmpx.so and answers the host-only queries (what a cgo / FFI binding relies on)."""
+    import shutil
+    import subprocess
+    from mpx import _abi
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "probe.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <string.h>
+#include "mpx.h"
+int main(void) {
+  mpx_variant_info info;
+  int n = mpx_variant_count();
+  if (mpx_query("nnedi3-nns256-win8x6", 1920, 1080, 3840, 2160, &info) != MPX_OK) return 2;
+  if (mpx_query("no-such-variant", 8, 8, 16, 16, &info) != MPX_ERR_UNKNOWN_VARIANT) return 3;
+  if (strstr(mpx_last_error(), "no-such-variant") == NULL) return 4;
+  printf("%d %d %d %.2f %.2f %s\n", n, info.out_w, info.out_h, info.offset_x, info.offset_y, info.hook);
+  return 0;
+}
+''')
+    exe = tmp_path / "probe"
+    libdir = os.path.dirname(_abi._LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-o", str(exe), str(src), "-I", os.path.join(ROOT, "include"),
+                    "-L", libdir, "-lmpx", "-Wl,-rpath," + libdir], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert out == ["46", "3840", "2160", "-0.50", "-0.50", "LUMA"]
