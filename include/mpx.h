@@ -170,6 +170,17 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
 int mpx_resample(const mpx_io* io, int in_w, int in_h, int in_type, int bits, int out_w, int out_h,
                  int out_type, int planes, int filter, float offset_x, float offset_y, void* hip_stream);
 
+/* Run the plan on n (1..MPX_MAX_FRAMES, <= max_batch) frames of DIFFERENT sizes in one
+ * launch per pass: frame i is widths[i] x heights[i] (each at most the plan's input size),
+ * rows in_pitches[i] / out_pitches[i] elements apart, planes contiguous; every frame is
+ * scaled by the plan's factors (the plan must apply: NNEDI3, ravu, ravu-lite, ravu-3x;
+ * not ravu-zoom, whose output size is the plan's target).  The kernels walk one tile
+ * queue across all frames.  No allocation.  Replaces mpv running the hook's passes on
+ * each frame at its own size. */
+int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* const* outs, const int* widths,
+                         const int* heights, const int64_t* in_pitches, const int64_t* out_pitches,
+                         void* hip_stream);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* mpx_last_error(void);
 

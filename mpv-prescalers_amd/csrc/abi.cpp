@@ -363,6 +363,50 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
   return dispatch(plan, d, static_cast<hipStream_t>(hip_stream));
 }
 
+int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* const* outs, const int* widths,
+                         const int* heights, const int64_t* in_pitches, const int64_t* out_pitches,
+                         void* hip_stream) {
+  if (!plan || !ins || !outs || !widths || !heights || !in_pitches || !out_pitches)
+    return fail(MPX_ERR_ARG, "NULL argument");
+  if (n < 1 || n > MPX_MAX_FRAMES || n > plan->max_batch)
+    return fail(MPX_ERR_SHAPE, "n = " + std::to_string(n) + " outside 1.." +
+                                   std::to_string(std::min(MPX_MAX_FRAMES, plan->max_batch)));
+  const mpx_variant_info& inf = plan->info;
+  if (!inf.applies) return fail(MPX_ERR_UNSUPPORTED, "frames of different sizes need a plan whose hook applies");
+  if (plan->v->family == MPX_FAMILY_RAVU_ZOOM)
+    return fail(MPX_ERR_UNSUPPORTED, "ravu-zoom scales to the plan's target size: one size per launch");
+  mpx::Io d{};
+  int64_t pixels = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!ins[i] || !outs[i]) return fail(MPX_ERR_ARG, "frame " + std::to_string(i) + " has a NULL plane");
+    if (widths[i] < 1 || heights[i] < 1 || widths[i] > plan->in_w || heights[i] > plan->in_h)
+      return fail(MPX_ERR_SHAPE, "frame " + std::to_string(i) + " is not within the plan's input size");
+    if (in_pitches[i] < widths[i] || out_pitches[i] < static_cast<int64_t>(widths[i]) * inf.scale_x)
+      return fail(MPX_ERR_ARG, "frame " + std::to_string(i) + ": pitch smaller than the row width");
+    d.tab_in[i] = ins[i];
+    d.tab_out[i] = outs[i];
+    d.fw[i] = widths[i];
+    d.fh[i] = heights[i];
+    d.fin_pitch[i] = in_pitches[i];
+    d.fout_pitch[i] = out_pitches[i];
+    pixels += static_cast<int64_t>(widths[i]) * inf.scale_x * heights[i] * inf.scale_y;
+  }
+  if (pixels > kMaxLaunchPixels) return fail(MPX_ERR_SHAPE, "more than 2^31 output pixels per plane in one launch");
+  mpx_host::DeviceScope dscope;
+  if (int drc = dscope.enter(plan->ctx->device)) return drc;
+  d.ntab = n;
+  d.sized = 1;
+  d.sx = inf.scale_x;
+  d.sy = inf.scale_y;
+  d.in_dt = plan->in_type;
+  d.out_dt = plan->out_type;
+  d.in_max = plan->in_max;
+  d.W = plan->in_w;
+  d.H = plan->in_h;
+  d.batch = n;
+  return dispatch(plan, d, static_cast<hipStream_t>(hip_stream));
+}
+
 static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st) {
   const mpx_variant_info& inf = plan->info;
   hipError_t e = hipSuccess;

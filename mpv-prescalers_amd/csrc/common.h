@@ -31,6 +31,21 @@ struct Io {
   float in_max;       // integer full-scale value for U8/U16 inputs (255, 1023, ...)
   int W, H;           // input plane size
   int batch;
+  // Frames of different sizes in one launch (sized != 0; ntab frames through the pointer
+  // tables): each frame's input size and row pitches, planes contiguous.  `tiles` is the
+  // prefix sum of the launch's tiles per frame, filled by the launcher for its tile shape.
+  int sized;
+  int fw[kMaxTab], fh[kMaxTab];
+  int64_t fin_pitch[kMaxTab], fout_pitch[kMaxTab];
+  int tiles[kMaxTab + 1];
+  int sx, sy;         // output scale of the launch (for the output plane stride)
+};
+
+// One tile of a launch: its frame, origin and that frame's geometry.
+struct FrameTile {
+  int f, x0, y0;
+  int W, H;
+  int64_t in_pitch, out_pitch, in_cstride, out_cstride;
 };
 
 __device__ __host__ __forceinline__ int elem_bytes(int dt) { return dt == DT_F32 ? 4 : (dt == DT_U8 ? 1 : 2); }
@@ -64,6 +79,51 @@ __device__ __forceinline__ BlockXYZ block_xyz(int nx, int ny, int nz) {
   const int t = xcd_tile(static_cast<int>(blockIdx.x), per * nz);
   const int z = t / per, r = t - z * per;
   return {r % nx, r / nx, z};
+}
+
+// The TX x TY input tile of block L (launch order, XCD-aware) among `total` tiles.
+template <int TX, int TY>
+__device__ __forceinline__ FrameTile frame_tile(const Io& io, int L, int total) {
+  const int t = xcd_tile(L, total);
+  FrameTile ft;
+  if (!io.sized) {
+    const int nx = (io.W + TX - 1) / TX, per = nx * ((io.H + TY - 1) / TY);
+    ft.f = t / per;
+    const int r = t - ft.f * per;
+    ft.x0 = (r % nx) * TX;
+    ft.y0 = (r / nx) * TY;
+    ft.W = io.W;
+    ft.H = io.H;
+    ft.in_pitch = io.in_pitch;
+    ft.out_pitch = io.out_pitch;
+    ft.in_cstride = io.in_cstride;
+    ft.out_cstride = io.out_cstride;
+    return ft;
+  }
+  int f = 0;
+  while (f + 1 < io.ntab && t >= io.tiles[f + 1]) ++f;
+  const int r = t - io.tiles[f];
+  const int nx = (io.fw[f] + TX - 1) / TX;
+  ft.f = f;
+  ft.x0 = (r % nx) * TX;
+  ft.y0 = (r / nx) * TY;
+  ft.W = io.fw[f];
+  ft.H = io.fh[f];
+  ft.in_pitch = io.fin_pitch[f];
+  ft.out_pitch = io.fout_pitch[f];
+  ft.in_cstride = io.fin_pitch[f] * io.fh[f];
+  ft.out_cstride = io.fout_pitch[f] * io.fh[f] * io.sy;
+  return ft;
+}
+
+// Host: tiles of a launch for a TX x TY tile (fills the per-frame prefix of sized launches).
+template <int TX, int TY>
+inline int launch_tiles(Io& io) {
+  if (!io.sized) return ((io.W + TX - 1) / TX) * ((io.H + TY - 1) / TY) * io.batch;
+  io.tiles[0] = 0;
+  for (int f = 0; f < io.ntab; ++f)
+    io.tiles[f + 1] = io.tiles[f] + ((io.fw[f] + TX - 1) / TX) * ((io.fh[f] + TY - 1) / TY);
+  return io.tiles[io.ntab];
 }
 
 __device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, float in_max) {

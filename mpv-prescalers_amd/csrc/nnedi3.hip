@@ -35,6 +35,9 @@ typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 typedef short short4v __attribute__((ext_vector_type(4)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
+constexpr int NX = 64;  // predicted pixels per block tile: NX x NY
+constexpr int NY = 4;
+
 struct NPass {
   const void* src;
   void* dst;
@@ -46,10 +49,46 @@ struct NPass {
   float src_max;
   int W, H;  // source plane
   int batch;
+  // frames of different sizes (sized != 0, ntab_src = ntab_dst frames): each frame's
+  // source size and row pitches; tiles = prefix of the pass's tiles per frame
+  int sized;
+  int fW[kMaxTab], fH[kMaxTab];
+  int64_t fsp[kMaxTab], fdp[kMaxTab];
+  int tiles[kMaxTab + 1];
 };
 
-constexpr int NX = 64;  // predicted pixels per block tile: NX x NY
-constexpr int NY = 4;
+// The NX x NY tile t of a pass: frame, origin and that frame's geometry.
+struct PassTile {
+  int f, x0, y0, W, H;
+  int64_t sp, dp;
+};
+__device__ __forceinline__ PassTile pass_tile(const NPass& p, int t) {
+  PassTile r;
+  if (!p.sized) {
+    const int nx = (p.W + NX - 1) / NX, per = nx * ((p.H + NY - 1) / NY);
+    r.f = t / per;
+    const int q = t - r.f * per;
+    r.x0 = (q % nx) * NX;
+    r.y0 = (q / nx) * NY;
+    r.W = p.W;
+    r.H = p.H;
+    r.sp = p.src_pitch;
+    r.dp = p.dst_pitch;
+    return r;
+  }
+  int f = 0;
+  while (f + 1 < p.ntab_src && t >= p.tiles[f + 1]) ++f;
+  const int q = t - p.tiles[f], nx = (p.fW[f] + NX - 1) / NX;
+  r.f = f;
+  r.x0 = (q % nx) * NX;
+  r.y0 = (q / nx) * NY;
+  r.W = p.fW[f];
+  r.H = p.fH[f];
+  r.sp = p.fsp[f];
+  r.dp = p.fdp[f];
+  return r;
+}
+
 constexpr float kVarEps = 1.192092896e-7f;  // nnedi3-nns16-win8x4.hook:28
 
 // Window / tile geometry.  AXIS 0 = double_y (new pixel between rows y and y+1), window
@@ -95,13 +134,14 @@ __device__ __forceinline__ void normalise(float* x, float& m, float& sd) {
 }
 
 template <int AXIS>
-__device__ __forceinline__ void store_result(const NPass& p, void* dst, int x, int y, float orig, float pred) {
+__device__ __forceinline__ void store_result(const NPass& p, void* dst, int64_t pitch, int x, int y, float orig,
+                                             float pred) {
   if (AXIS == 0) {  // combine_y: row 2y = input row y, row 2y+1 = prediction
-    const int64_t o = static_cast<int64_t>(2 * y) * p.dst_pitch + x;
+    const int64_t o = static_cast<int64_t>(2 * y) * pitch + x;
     store_elem(dst, o, p.dst_dt, orig);
-    store_elem(dst, o + p.dst_pitch, p.dst_dt, pred);
+    store_elem(dst, o + pitch, p.dst_dt, pred);
   } else {          // combine_x: column 2x = input, 2x+1 = prediction
-    store_pair(dst, static_cast<int64_t>(y) * p.dst_pitch + 2 * x, p.dst_dt, orig, pred);
+    store_pair(dst, static_cast<int64_t>(y) * pitch + 2 * x, p.dst_dt, orig, pred);
   }
 }
 
@@ -117,22 +157,22 @@ __device__ __forceinline__ void* frame_dst(const NPass& p, int f) {
 // ----------------------------------------------------------------------------- FP32 VALU
 
 template <int WH, int AXIS>
-__global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restrict__ wt, int nns) {
+__global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restrict__ wt, int nns, int total) {
   using G = Geo<WH, AXIS>;
   constexpr int K = G::K;
   __shared__ float tile[G::TH * G::TW];
-  const BlockXYZ bk3 = block_xyz((p.W + NX - 1) / NX, (p.H + NY - 1) / NY, p.batch);
-  const int f = bk3.z;
-  const int x0 = bk3.x * NX, y0 = bk3.y * NY;
+  const PassTile pt = pass_tile(p, xcd_tile(blockIdx.x, total));
+  const int f = pt.f;
+  const int x0 = pt.x0, y0 = pt.y0;
   {
     NTile<WH, AXIS> ht;
-    ht.issue(frame_src(p, f), p.src_dt, p.src_pitch, 0, p.W, p.H, x0 + G::OX, y0 + G::OY);
+    ht.issue(frame_src(p, f), p.src_dt, pt.sp, 0, pt.W, pt.H, x0 + G::OX, y0 + G::OY);
     ht.commit(tile, p.src_dt, p.src_max);
   }
   __syncthreads();
   const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;
   const int x = x0 + tx, y = y0 + ty;
-  if (x >= p.W || y >= p.H) return;
+  if (x >= pt.W || y >= pt.H) return;
   float xv[K];
 #pragma unroll
   for (int a = 0; a < 8; ++a)
@@ -155,7 +195,7 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
     vsum = fmaf(e, q, vsum);
   }
   const float pred = fminf(fmaxf(m + 5.0f * vsum * __builtin_amdgcn_rcpf(wsum) * sd, 0.f), 1.f);
-  store_result<AXIS>(p, frame_dst(p, f), x, y, tile[G::centre(tx, ty)], pred);
+  store_result<AXIS>(p, frame_dst(p, f), pt.dp, x, y, tile[G::centre(tx, ty)], pred);
 }
 
 // ----------------------------------------------------------------------------- MFMA
@@ -332,7 +372,7 @@ template <int WH, int AXIS, int FORM, int TERMS, int NSL, int TPW, bool SCALED>
 __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __restrict__ w_hi,
                                                       const void* __restrict__ w_lo, const int* __restrict__ wsc,
                                                       const float* __restrict__ bias, float xscale, float inv_acc,
-                                                      int tiles_x, int tiles_y, int total_tiles) {
+                                                      int total_tiles) {
   using G = Geo<WH, AXIS>;
   constexpr int K = G::K;
   using M = Mma<FORM, K, TERMS>;
@@ -398,21 +438,19 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     vs += __shfl_xor(vs, 32);
   };
 
-  const int per_frame = tiles_x * tiles_y;
   NTile<WH, AXIS> ht;
   auto issue = [&](int id) {
-    const int ff = id / per_frame, rr = id - ff * per_frame;
-    ht.issue(frame_src(p, ff), p.src_dt, p.src_pitch, 0, p.W, p.H, (rr % tiles_x) * NX + G::OX,
-             (rr / tiles_x) * NY + G::OY);
+    const PassTile q = pass_tile(p, id);
+    ht.issue(frame_src(p, q.f), p.src_dt, q.sp, 0, q.W, q.H, q.x0 + G::OX, q.y0 + G::OY);
   };
   // launch-order index L = blockIdx + i*gridDim walks [0, total); tile = xcd_tile(L)
   if (static_cast<int>(blockIdx.x) < total_tiles) issue(xcd_tile(blockIdx.x, total_tiles));
   int buf = 0;
   for (int L = blockIdx.x; L < total_tiles; L += gridDim.x, buf ^= 1) {
     const int tileid = xcd_tile(L, total_tiles);
-    const int f = tileid / per_frame;
-    const int rem = tileid - f * per_frame;
-    const int y0 = (rem / tiles_x) * NY, x0 = (rem % tiles_x) * NX;
+    const PassTile pt = pass_tile(p, tileid);
+    const int f = pt.f;
+    const int y0 = pt.y0, x0 = pt.x0;
     // The other buffer was last read two tiles ago, before the barrier below was last
     // passed by every wave.
     ht.commit(tile[buf], p.src_dt, p.src_max);
@@ -468,9 +506,9 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     }
 
     const int x = x0 + tx, y = y0 + ty;
-    if (x < p.W && y < p.H) {
+    if (x < pt.W && y < pt.H) {
       const float pred = fminf(fmaxf(m + 5.0f * vsum * __builtin_amdgcn_rcpf(wsum) * sd, 0.f), 1.f);
-      store_result<AXIS>(p, frame_dst(p, f), x, y, orig, pred);
+      store_result<AXIS>(p, frame_dst(p, f), pt.dp, x, y, orig, pred);
     }
     if constexpr (NSL == 1) wave_lds_sync();  // this tile's row reads before the next tile's writes
   }
@@ -479,14 +517,18 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
 // ----------------------------------------------------------------------------- launcher
 
 template <int WH, int AXIS>
-static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t st) {
-  const int tiles_x = (p.W + NX - 1) / NX, tiles_y = (p.H + NY - 1) / NY;
+static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
+  int total = ((p.W + NX - 1) / NX) * ((p.H + NY - 1) / NY) * p.batch;
+  if (p.sized) {
+    p.tiles[0] = 0;
+    for (int f = 0; f < p.ntab_src; ++f) p.tiles[f + 1] = p.tiles[f] + ((p.fW[f] + NX - 1) / NX) * ((p.fH[f] + NY - 1) / NY);
+    total = p.tiles[p.ntab_src];
+  }
   if (nw.precision == 0) {
-    dim3 grid(tiles_x * tiles_y * p.batch);
-    hipLaunchKernelGGL((nnedi3_valu<WH, AXIS>), grid, dim3(256), 0, st, p, static_cast<const float*>(nw.w), nw.nns);
+    hipLaunchKernelGGL((nnedi3_valu<WH, AXIS>), dim3(total), dim3(256), 0, st, p, static_cast<const float*>(nw.w),
+                       nw.nns, total);
     return hipGetLastError();
   }
-  const int total = tiles_x * tiles_y * p.batch;
   const int nt = nw.nns / 16;
   const float inv_acc = 1.0f / (nw.xscale * nw.wscale);
   // Neuron tiles are split over NSL waves only when one wave cannot hold them all
@@ -502,7 +544,7 @@ static hipError_t run_pass(const NnediWeights& nw, const NPass& p, hipStream_t s
     const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
     const int grid = min(total, device_cu_count() * k);                                                      \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
-                       inv_acc, tiles_x, tiles_y, total);                                             \
+                       inv_acc, total);                                                               \
   }
 #define MPX_NN_NT(FM, TERMS, SC)                    \
   switch (nt) {                                     \
@@ -564,6 +606,17 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
       p.dst_bstride = io.out_bstride;
       p.dst_dt = io.out_dt;
     }
+    if (io.sized) {  // frames of different sizes: per-frame geometry, intermediates through tables
+      p.sized = 1;
+      p.ntab_dst = io.ntab;
+      for (int f = 0; f < io.ntab; ++f) {
+        p.fW[f] = io.fw[f];
+        p.fH[f] = io.fh[f];
+        p.fsp[f] = io.fin_pitch[f];
+        p.fdp[f] = dx ? io.fw[f] : io.fout_pitch[f];
+        p.tab_dst[f] = dx ? static_cast<void*>(tmp + f * tmp_frame) : io.tab_out[f];
+      }
+    }
     e = nw.wh == 4 ? run_pass<4, 0>(nw, p, st) : run_pass<6, 0>(nw, p, st);
     if (e != hipSuccess) return e;
   }
@@ -598,6 +651,17 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
     p.dst_pitch = io.out_pitch;
     p.dst_bstride = io.out_bstride;
     p.dst_dt = io.out_dt;
+    if (io.sized) {
+      p.sized = 1;
+      p.ntab_src = io.ntab;
+      for (int f = 0; f < io.ntab; ++f) {
+        p.fW[f] = io.fw[f];
+        p.fH[f] = dy ? 2 * io.fh[f] : io.fh[f];
+        p.fsp[f] = dy ? io.fw[f] : io.fin_pitch[f];
+        p.fdp[f] = io.fout_pitch[f];
+        p.tab_src[f] = dy ? static_cast<const void*>(tmp + f * tmp_frame) : io.tab_in[f];
+      }
+    }
     e = nw.wh == 4 ? run_pass<4, 1>(nw, p, st) : run_pass<6, 1>(nw, p, st);
   }
   return e;

@@ -50,7 +50,7 @@ __device__ __forceinline__ double luma_d(const float* t, int o, int n) {
 // ----------------------------------------------------------------------------- RAVU-3x
 
 template <int R, int CH>
-__global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __restrict__ lut) {
+__global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __restrict__ lut, int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R - 1;
   constexpr int HALO = R - 1;
@@ -58,20 +58,20 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   constexpr int C = (N * N - 1) / 2;
   constexpr int NTEX = 2 * (C + 1);
   __shared__ float tile[P * PN];
-  const BlockXYZ bk3 = block_xyz((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
-  const int f = bk3.z;
-  const int x0 = bk3.x * RT_X, y0 = bk3.y * RT_Y;
+  const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
+  const int f = ft.f;
+  const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - HALO, y0 - HALO);
+  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - HALO, y0 - HALO);
   __syncthreads();
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
-  if (x >= io.W || y >= io.H) return;
+  if (x >= ft.W || y >= ft.H) return;
   auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };  // sample i = X*N + Y
   const Bucket bk = ravu_hash<N, false, RULE_3X>([&](int i) { return luma_f<CH>(tile, off(i), PN); },
                                                  [&](int i) { return luma_d<CH>(tile, off(i), PN); });
   const int row = bucket_row<RULE_3X>(bk, 216);
-  if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * io.W + x] = row;
+  if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * ft.W + x] = row;
   const __half* lrow = lut + static_cast<int64_t>(row) * NTEX * 4;
   float r0[P][4], r1[P][4];
 #pragma unroll
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   char* ob = static_cast<char*>(frame_out(io, f));
 #pragma unroll
   for (int c = 0; c < P; ++c) {
-    void* o = ob + c * io.out_cstride * esz_out;
+    void* o = ob + c * ft.out_cstride * esz_out;
     const float centre = tile[c * PN + off(C)];
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
@@ -114,16 +114,18 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
         const int k = 3 * dx + dy;  // compute/ravu-3x-r3.hook:154-162
         float v = k < 4 ? r0[c][k] : (k == 4 ? centre : r1[c][k - 5]);
         if (k != 4) v = fminf(fmaxf(v, 0.f), 1.f);
-        store_elem(o, static_cast<int64_t>(3 * y + dy) * io.out_pitch + 3 * x + dx, io.out_dt, v);
+        store_elem(o, static_cast<int64_t>(3 * y + dy) * ft.out_pitch + 3 * x + dx, io.out_dt, v);
       }
   }
 }
 
-hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, hipStream_t st) {
-  dim3 grid(((io.W + RT_X - 1) / RT_X) * ((io.H + RT_Y - 1) / RT_Y) * io.batch);
+hipError_t launch_ravu_3x(int r, int channels, const Io& io_, const __half* lut, hipStream_t st) {
+  Io io = io_;
+  const int total = launch_tiles<RT_X, RT_Y>(io);
+  dim3 grid(total);
 #define MPX_3X(R, CH)                                                                                  \
   if (r == R && channels == CH) {                                                                      \
-    hipLaunchKernelGGL((ravu_3x_kernel<R, CH>), grid, dim3(256), 0, st, io, lut);                      \
+    hipLaunchKernelGGL((ravu_3x_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, total);               \
     return hipGetLastError();                                                                          \
   }
   MPX_3X(2, 0) MPX_3X(2, 1) MPX_3X(2, 2) MPX_3X(3, 0) MPX_3X(3, 1) MPX_3X(3, 2) MPX_3X(4, 0) MPX_3X(4, 1) MPX_3X(4, 2)
@@ -167,31 +169,31 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
 // step1: int11 from the N = 2R window at offsets -(R-1)..R (ravu-r3.hook:22-146).
 template <int R, int CH>
 __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
-                                                         int64_t int11_frame) {
+                                                         int64_t int11_frame, int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
   constexpr int PW = RT_X + N - 1, PH = RT_Y + N - 1, PN = PW * PH;
   __shared__ float tile[P * PN];
-  const BlockXYZ bk3 = block_xyz((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
-  const int f = bk3.z;
-  const int x0 = bk3.x * RT_X, y0 = bk3.y * RT_Y;
+  const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
+  const int f = ft.f;
+  const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_pitch, io.in_cstride, io.W, io.H, x0 - (R - 1),
+  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
                   y0 - (R - 1));
   __syncthreads();
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
-  if (x >= io.W || y >= io.H) return;
+  if (x >= ft.W || y >= ft.H) return;
   auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };
   const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(tile, off(i), PN); },
                                                   [&](int i) { return luma_d<CH>(tile, off(i), PN); });
   const int row = bucket_row<RULE_LOG2>(bk, 648);
-  if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * io.W + x] = row;
+  if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * ft.W + x] = row;
   float res[P];
   ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4,
                   [&](int c, int i) { return tile[c * PN + off(i)]; }, res);
-  const int64_t plane = static_cast<int64_t>(io.W) * io.H;
-  float* o = int11 + f * int11_frame + static_cast<int64_t>(y) * io.W + x;
+  const int64_t plane = static_cast<int64_t>(ft.W) * ft.H;
+  float* o = int11 + f * int11_frame + static_cast<int64_t>(y) * ft.W + x;
 #pragma unroll
   for (int c = 0; c < P; ++c) o[c * plane] = res[c];
 }
@@ -210,28 +212,29 @@ __device__ __host__ constexpr int lat2y(int X, int Y) { return STEP == 2 ? Y - X
 // pairs and each holds one prediction's registers only.
 template <int R, int CH, int STEP>
 __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* __restrict__ lut,
-                                                           const float* __restrict__ int11, int64_t int11_frame) {
+                                                           const float* __restrict__ int11, int64_t int11_frame,
+                                                           int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
   constexpr int PW = RT_X + 2 * R + 1, PH = RT_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
   __shared__ float hk[P * PN];   // input (HOOKED)
   __shared__ float i11[P * PN];  // ravu_int11
-  const BlockXYZ bk3 = block_xyz((io.W + RT_X - 1) / RT_X, (io.H + RT_Y - 1) / RT_Y, io.batch);
-  const int f = bk3.z;
-  const int x0 = bk3.x * RT_X, y0 = bk3.y * RT_Y;
+  const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
+  const int f = ft.f;
+  const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  const int64_t plane = static_cast<int64_t>(io.W) * io.H;
+  const int64_t plane = static_cast<int64_t>(ft.W) * ft.H;
   {
     HaloTile<P, PH, PW, 256> a, b;  // both tiles' loads in flight together
-    a.issue(in, io.in_dt, io.in_pitch, io.in_cstride, io.W, io.H, x0 - R, y0 - R);
-    b.issue(int11 + f * int11_frame, DT_F32, io.W, plane, io.W, io.H, x0 - R, y0 - R);
+    a.issue(in, io.in_dt, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - R, y0 - R);
+    b.issue(int11 + f * int11_frame, DT_F32, ft.W, plane, ft.W, ft.H, x0 - R, y0 - R);
     a.commit(hk, io.in_dt, io.in_max);
     b.commit(i11, DT_F32, 1.f);
   }
   __syncthreads();
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
-  if (x >= io.W || y >= io.H) return;
+  if (x >= ft.W || y >= ft.H) return;
   const int base = (ty + R) * PW + tx + R;  // LDS offset of (x, y)
   auto src = [&](int i) -> const float* {
     const int X = i / N, Y = i % N;
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
   const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
                                                   [&](int i) { return luma_d<CH>(src(i), 0, PN); });
   const int row = bucket_row<RULE_LOG2>(bk, 648);
-  if (io.buckets) io.buckets[f * io.bucket_bstride + (STEP - 1) * plane + static_cast<int64_t>(y) * io.W + x] = row;
+  if (io.buckets) io.buckets[f * io.bucket_bstride + (STEP - 1) * plane + static_cast<int64_t>(y) * ft.W + x] = row;
   float res[P];
   ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
                   res);
@@ -249,26 +252,28 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
   char* ob = static_cast<char*>(frame_out(io, f));
 #pragma unroll
   for (int c = 0; c < P; ++c) {
-    void* o = ob + c * io.out_cstride * esz_out;
+    void* o = ob + c * ft.out_cstride * esz_out;
     if (STEP == 2)
-      store_pair(o, static_cast<int64_t>(2 * y) * io.out_pitch + 2 * x, io.out_dt, hk[c * PN + base], res[c]);
+      store_pair(o, static_cast<int64_t>(2 * y) * ft.out_pitch + 2 * x, io.out_dt, hk[c * PN + base], res[c]);
     else
-      store_pair(o, static_cast<int64_t>(2 * y + 1) * io.out_pitch + 2 * x, io.out_dt, res[c], i11[c * PN + base]);
+      store_pair(o, static_cast<int64_t>(2 * y + 1) * ft.out_pitch + 2 * x, io.out_dt, res[c], i11[c * PN + base]);
   }
 }
 
-hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
+hipError_t launch_ravu(int r, int channels, const Io& io_, const __half* lut, float* int11, int64_t int11_frame,
                        hipStream_t st) {
-  dim3 grid(((io.W + RT_X - 1) / RT_X) * ((io.H + RT_Y - 1) / RT_Y) * io.batch);
+  Io io = io_;
+  const int total = launch_tiles<RT_X, RT_Y>(io);
+  dim3 grid(total);
 #define MPX_RAVU(R, CH)                                                                                        \
   if (r == R && channels == CH) {                                                                              \
-    hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);       \
+    hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame, total);       \
     hipError_t e = hipGetLastError();                                                                          \
     if (e != hipSuccess) return e;                                                                             \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);  \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2>), grid, dim3(256), 0, st, io, lut, int11, int11_frame, total);  \
     e = hipGetLastError();                                                                                     \
     if (e != hipSuccess) return e;                                                                             \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3>), grid, dim3(256), 0, st, io, lut, int11, int11_frame);  \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3>), grid, dim3(256), 0, st, io, lut, int11, int11_frame, total);  \
     return hipGetLastError();                                                                                  \
   }
   MPX_RAVU(2, 0) MPX_RAVU(2, 1) MPX_RAVU(2, 2) MPX_RAVU(3, 0) MPX_RAVU(3, 1) MPX_RAVU(3, 2) MPX_RAVU(4, 0)

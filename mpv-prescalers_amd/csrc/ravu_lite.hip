@@ -19,7 +19,7 @@ constexpr int LITE_TX = 64;
 constexpr int LITE_TY = 4;
 
 template <int R, bool AR>
-__global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __restrict__ lut) {
+__global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __restrict__ lut, int total) {
   constexpr int N = 2 * R - 1;
   constexpr int HALO = R - 1;
   constexpr int PW = LITE_TX + 2 * HALO;
@@ -28,15 +28,15 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   constexpr int C = (N * N - 1) / 2;
   __shared__ float tile[PH * PW];
 
-  const BlockXYZ bk3 = block_xyz((io.W + LITE_TX - 1) / LITE_TX, (io.H + LITE_TY - 1) / LITE_TY, io.batch);
-  const int frame = bk3.z;
-  const int x0 = bk3.x * LITE_TX;
-  const int y0 = bk3.y * LITE_TY;
+  const FrameTile ft = frame_tile<LITE_TX, LITE_TY>(io, blockIdx.x, total);
+  const int frame = ft.f;
+  const int x0 = ft.x0;
+  const int y0 = ft.y0;
   const void* in = frame_in(io, frame);
 
   {
     HaloTile<1, PH, PW, 256> h;
-    h.issue(in, io.in_dt, io.in_pitch, 0, io.W, io.H, x0 - HALO, y0 - HALO);
+    h.issue(in, io.in_dt, ft.in_pitch, 0, ft.W, ft.H, x0 - HALO, y0 - HALO);
     h.commit(tile, io.in_dt, io.in_max);
   }
   __syncthreads();
@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   const int tx = threadIdx.x % LITE_TX;
   const int ty = threadIdx.x / LITE_TX;
   const int x = x0 + tx, y = y0 + ty;
-  if (x >= io.W || y >= io.H) return;
+  if (x >= ft.W || y >= ft.H) return;
 
   // window sample i = X*N + Y at offset (X - HALO, Y - HALO)
   float s[N * N];
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
       [&](int i) { return s[i]; }, [&](int i) { return static_cast<double>(s[i]); });
   const int row = bucket_row<RULE_LITE>(bk, 288);
   if (io.buckets)
-    io.buckets[frame * io.bucket_bstride + static_cast<int64_t>(y) * io.W + x] = row;
+    io.buckets[frame * io.bucket_bstride + static_cast<int64_t>(y) * ft.W + x] = row;
 
   const __half* lrow = lut + static_cast<int64_t>(row) * NTEX * 4;
   float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -125,16 +125,18 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   }
   // res[2*dx + dy] -> output (2x+dx, 2y+dy)
   void* out = frame_out(io, frame);
-  const int64_t o0 = static_cast<int64_t>(2 * y) * io.out_pitch + 2 * x;
+  const int64_t o0 = static_cast<int64_t>(2 * y) * ft.out_pitch + 2 * x;
   store_pair(out, o0, io.out_dt, r4[0], r4[2]);
-  store_pair(out, o0 + io.out_pitch, io.out_dt, r4[1], r4[3]);
+  store_pair(out, o0 + ft.out_pitch, io.out_dt, r4[1], r4[3]);
 }
 
-hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, hipStream_t st) {
-  dim3 grid(((io.W + LITE_TX - 1) / LITE_TX) * ((io.H + LITE_TY - 1) / LITE_TY) * io.batch);
+hipError_t launch_ravu_lite(int r, bool ar, const Io& io_, const __half* lut, hipStream_t st) {
+  Io io = io_;
+  const int total = launch_tiles<LITE_TX, LITE_TY>(io);
+  dim3 grid(total);
   dim3 block(LITE_TX * LITE_TY);
 #define MPX_LITE(R, A) \
-  if (r == R && ar == A) { hipLaunchKernelGGL((ravu_lite_kernel<R, A>), grid, block, 0, st, io, lut); return hipGetLastError(); }
+  if (r == R && ar == A) { hipLaunchKernelGGL((ravu_lite_kernel<R, A>), grid, block, 0, st, io, lut, total); return hipGetLastError(); }
   MPX_LITE(2, false) MPX_LITE(2, true) MPX_LITE(3, false) MPX_LITE(3, true) MPX_LITE(4, false) MPX_LITE(4, true)
 #undef MPX_LITE
   return hipErrorInvalidValue;

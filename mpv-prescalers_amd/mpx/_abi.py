@@ -24,7 +24,7 @@ CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 # every exported symbol declared in include/mpx.h
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
-           "mpx_run_ex", "mpx_run_frames", "mpx_plan_tune", "mpx_resample", "mpx_last_error")
+           "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_resample", "mpx_last_error")
 MAX_FRAMES = 16
 
 
@@ -76,6 +76,8 @@ def _load():
         "mpx_run": (i32, [vp, vp, i64, i64, vp, i64, i64, i32, vp]),
         "mpx_run_ex": (i32, [vp, ctypes.POINTER(MpxIo), vp]),
         "mpx_run_frames": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), i64, i64, vp]),
+        "mpx_run_frames_sized": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i32),
+                                       ctypes.POINTER(i32), ctypes.POINTER(i64), ctypes.POINTER(i64), vp]),
         "mpx_plan_tune": (i32, [vp, ctypes.POINTER(MpxIo), vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                 ctypes.POINTER(ctypes.c_float)]),
         "mpx_resample": (i32, [ctypes.POINTER(MpxIo), i32, i32, i32, i32, i32, i32, i32, i32, i32,

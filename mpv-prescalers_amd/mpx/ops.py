@@ -159,6 +159,37 @@ class Plan:
         return outs
 
 
+    def frame_out_shape(self, x):
+        """Output shape of one frame ``x`` of any size within the plan (run_frames_sized)."""
+        lead = tuple(x.shape[:-2])
+        return lead + (x.shape[-2] * self.info["scale_y"], x.shape[-1] * self.info["scale_x"])
+
+    def run_frames_sized(self, xs, outs=None, stream=None):
+        """One launch (per pass) over frames of different sizes (up to 16, each within the
+        plan's input size, planes contiguous): every frame is scaled by the plan's factors
+        through one tile queue.  Not for ravu-zoom plans."""
+        n = len(xs)
+        if n < 1 or n > min(_abi.MAX_FRAMES, self.max_batch):
+            raise ValueError("run_frames_sized takes 1..%d frames" % min(_abi.MAX_FRAMES, self.max_batch))
+        if outs is None:
+            outs = [torch.empty(self.frame_out_shape(x), dtype=self.out_dtype, device=x.device) for x in xs]
+        for x, o in zip(xs, outs):
+            if x.dtype != self.in_dtype or not x.is_contiguous() or not o.is_contiguous() or o.dtype != self.out_dtype:
+                raise ValueError("frame does not match the plan")
+            if tuple(o.shape) != self.frame_out_shape(x):
+                raise ValueError("bad output shape")
+        ins = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+        ots = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        ws = (ctypes.c_int * n)(*[x.shape[-1] for x in xs])
+        hs = (ctypes.c_int * n)(*[x.shape[-2] for x in xs])
+        ips = (ctypes.c_int64 * n)(*[x.stride(-2) for x in xs])
+        ops_ = (ctypes.c_int64 * n)(*[o.stride(-2) for o in outs])
+        if stream is None:
+            stream = torch.cuda.current_stream(xs[0].device)
+        check(LIB.mpx_run_frames_sized(self._h, n, ins, ots, ws, hs, ips, ops_, ctypes.c_void_p(stream.cuda_stream)))
+        return outs
+
+
 class Context:
     """Device binding + resident weights + HBM budget (bytes, 0 = unlimited)."""
 

@@ -1,0 +1,45 @@
+"""Frames of different sizes in one launch (mpx_run_frames_sized): every frame equals the
+same variant run on that frame alone, bit for bit."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(200, 120), (37, 23), (128, 64), (5, 3), (199, 119), (64, 4)]
+
+
+def _frames(planes, seed):
+    from mpx.frames import make_frame
+    return [torch.from_numpy(make_frame(w, h, seed + i, bits=10, channels=planes).astype(np.float16)).cuda()
+            for i, (w, h) in enumerate(SIZES)]
+
+
+@pytest.mark.parametrize("name,prec", [("nnedi3-nns32-win8x4", "fp16"), ("nnedi3-nns64-win8x6", "fp32"),
+                                       ("nnedi3-nns128-win8x6", "fp16"), ("ravu-lite-ar-r3", "fp32"),
+                                       ("ravu-r3-yuv", "fp32"), ("ravu-r4", "fp32"), ("ravu-3x-r3-rgb", "fp32"),
+                                       ("ravu-3x-r2", "fp32")])
+def test_sized_launch_equals_single_frames(name, prec):
+    import mpx
+    planes = 3 if name.endswith(("-yuv", "-rgb")) else 1
+    frames = _frames(planes, 40)
+    plan = mpx.Context(0).plan(name, 200, 120, in_dtype=torch.float16, out_dtype=torch.float16, precision=prec,
+                               max_batch=len(frames))
+    outs = plan.run_frames_sized(frames)
+    for (w, h), x, o in zip(SIZES, frames, outs):
+        alone = mpx.upscale(name, x, precision=prec)
+        assert o.shape == alone.shape, (w, h)
+        assert torch.equal(o, alone), (name, w, h)
+
+
+def test_sized_launch_rejections():
+    import mpx
+    from mpx import MpxError
+    ctx = mpx.Context(0)
+    zoom = ctx.plan("ravu-zoom-r2", 64, 48, target=(96, 72), max_batch=2)
+    x = torch.rand((48, 64), device="cuda")
+    with pytest.raises(MpxError):
+        zoom.run_frames_sized([x])
+    lite = ctx.plan("ravu-lite-r2", 64, 48, max_batch=2)
+    with pytest.raises(MpxError):
+        lite.run_frames_sized([torch.rand((49, 64), device="cuda")])   # taller than the plan
