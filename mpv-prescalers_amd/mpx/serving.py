@@ -3,10 +3,12 @@
 Requests (a device frame + variant name + display size) arrive at any time.  A scheduler
 thread keeps the weights resident (one Context), admits queued requests in arrival order
 while the bytes of in-flight outputs stay within the HBM budget, groups the admitted ones
-by plan (variant, size, dtypes, precision), and launches each group -- up to 16 frames at
-independent addresses per launch (``mpx_run_frames``) -- on the least recently used of
-several HIP streams, so launches of different variants and sizes overlap on the device
-while new arrivals keep being admitted.  A completion thread waits on each launch's event
+by plan, and launches each group -- up to 16 frames at independent addresses per launch --
+on the least recently used of several HIP streams, so launches of different variants
+overlap on the device while new arrivals keep being admitted.  Frames at their variant's
+default display size share launches across sizes (``mpx_run_frames_sized``: one tile
+queue over frames of different sizes, plans of a fixed capacity); other requests group by
+variant and size (``mpx_run_frames``).  A completion thread waits on each launch's event
 in order and stamps every request's finish time; latency = finish - arrival.
 
 Each (plan key, stream) pair owns its plan, since a plan's scratch serves one stream at a
@@ -64,13 +66,17 @@ class Server:
     ``admit_bytes`` for the outputs of admitted, unfinished requests."""
 
     def __init__(self, device=0, hbm_budget=16 << 30, admit_bytes=None, streams=4, max_batch=8,
-                 plan_batch=None):
+                 plan_batch=None, mixed_sizes=(2560, 1440)):
         self.device = torch.device("cuda", device) if isinstance(device, int) else device
         self.ctx = Context(self.device.index or 0, hbm_budget=hbm_budget)
         self.streams = [torch.cuda.Stream(self.device) for _ in range(streams)]
         self.max_batch = max(1, min(16, max_batch))
         self.plan_batch = plan_batch or self.max_batch
         self.admit_bytes = admit_bytes if admit_bytes is not None else hbm_budget // 2
+        # frames up to this size (width, height) with the variant's default display size
+        # share launches across sizes (one plan of this capacity per variant); None: one
+        # plan and launch group per size
+        self.mixed_sizes = tuple(mixed_sizes) if mixed_sizes else None
         self._plans = collections.OrderedDict()   # (plan key, stream) -> Plan, LRU order
         self._last_use = {}                        # (plan key, stream) -> event of its last launch
         self._queue = collections.deque()
@@ -124,6 +130,17 @@ class Server:
         self.ctx.close()
 
     # ------------------------------------------------------------------ scheduler
+    def _key(self, r):
+        """Plan key of a request: mixed-size launches group by variant (plan of the
+        mixed_sizes capacity), the rest by variant and size."""
+        in_h, in_w = r.frame.shape[-2:]
+        if (self.mixed_sizes and not r.name.startswith("ravu-zoom") and in_w <= self.mixed_sizes[0]
+                and in_h <= self.mixed_sizes[1] and r.target == default_target(r.name, in_w, in_h)):
+            cw, ch = self.mixed_sizes
+            return (r.name, cw, ch, default_target(r.name, cw, ch), r.frame.dtype, r.out_dtype, r.precision, r.bits,
+                    True)
+        return (r.name, in_w, in_h, r.target, r.frame.dtype, r.out_dtype, r.precision, r.bits, False)
+
     def _plan(self, key, si):
         """The plan of (key, stream), created on first use.  When the context's HBM budget
         is exhausted, idle plans (their last launch finished) are evicted, oldest first."""
@@ -131,7 +148,7 @@ class Server:
         if p is not None:
             self._plans.move_to_end((key, si))
             return p
-        name, in_w, in_h, tgt, in_dt, out_dt, prec, bits = key
+        name, in_w, in_h, tgt, in_dt, out_dt, prec, bits, _ = key
         while True:
             try:
                 p = self.ctx.plan(name, in_w, in_h, tgt, in_dt, out_dt, prec, self.plan_batch, bits)
@@ -162,10 +179,9 @@ class Server:
         while self._queue:
             r = self._queue[0]
             if r.key is None:
-                in_h, in_w = r.frame.shape[-2:]
-                key = (r.name, in_w, in_h, r.target, r.frame.dtype, r.out_dtype, r.precision, r.bits)
+                key = self._key(r)
                 try:
-                    info = self._plan(key, 0).info
+                    plan = self._plan(key, 0)
                 except Exception as e:  # unknown / unavailable variant, bad size, budget
                     self._queue.popleft()
                     r.error = e
@@ -173,7 +189,8 @@ class Server:
                     r._done.set()
                     continue
                 r.key = key
-                r.bytes = info["planes"] * info["out_w"] * info["out_h"] * torch.finfo(r.out_dtype).bits // 8
+                shape = plan.frame_out_shape(r.frame) if key[-1] else plan.out_shape(r.frame)
+                r.bytes = int(torch.Size(shape).numel()) * torch.finfo(r.out_dtype).bits // 8
             if self._inflight + r.bytes > self.admit_bytes and self._inflight > 0:
                 break
             self._queue.popleft()
@@ -210,8 +227,13 @@ class Server:
             for src in {r.src_stream for r in reqs}:
                 st.wait_stream(src)  # the frames were produced on the submitters' streams
             with torch.cuda.stream(st):
-                outs = [torch.empty(plan.out_shape(r.frame), dtype=r.out_dtype, device=self.device) for r in reqs]
-                plan.run_frames([r.frame for r in reqs], outs, stream=st)
+                if key[-1]:   # frames of several sizes, one tile queue
+                    outs = [torch.empty(plan.frame_out_shape(r.frame), dtype=r.out_dtype, device=self.device)
+                            for r in reqs]
+                    plan.run_frames_sized([r.frame for r in reqs], outs, stream=st)
+                else:
+                    outs = [torch.empty(plan.out_shape(r.frame), dtype=r.out_dtype, device=self.device) for r in reqs]
+                    plan.run_frames([r.frame for r in reqs], outs, stream=st)
                 ev = torch.cuda.Event()
                 ev.record(st)
             self._last_use[(key, si)] = ev

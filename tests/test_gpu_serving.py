@@ -96,7 +96,7 @@ def test_plan_eviction_under_context_budget():
     import mpx
     sizes = [(256, 256), (200, 200), (160, 160), (256, 256)]
     budget = 64 * 1024 + 256 * 256 * 4 + 32 * 1024   # weights + the largest int11 scratch + slack
-    srv = Server(device=0, hbm_budget=budget, admit_bytes=1 << 30, streams=1, max_batch=1)
+    srv = Server(device=0, hbm_budget=budget, admit_bytes=1 << 30, streams=1, max_batch=1, mixed_sizes=None)
     try:
         frames = [_frame(w, h, 1, 30 + i) for i, (w, h) in enumerate(sizes)]
         rs = [srv.submit("ravu-r2", f) for f in frames]
@@ -104,3 +104,24 @@ def test_plan_eviction_under_context_budget():
             assert torch.equal(r.wait(timeout=60), mpx.upscale("ravu-r2", f))
     finally:
         srv.close()
+
+
+def test_mixed_sizes_share_launches():
+    """Frames of different sizes of one variant go out in shared launches (one tile queue
+    over all of them) and each equals its result alone."""
+    from mpx.serving import Server
+    import mpx
+    sizes = [(160, 90), (128, 72), (200, 120), (64, 36), (96, 54), (150, 100)]
+    srv = Server(device=0, hbm_budget=4 << 30, streams=1, max_batch=8, mixed_sizes=(256, 256))
+    try:
+        srv.pause()
+        frames = [_frame(w, h, 1, 70 + i) for i, (w, h) in enumerate(sizes)]
+        rs = [srv.submit("ravu-lite-ar-r3", f) for f in frames]
+        before = srv.launches
+        srv.resume()
+        for f, r in zip(frames, rs):
+            assert torch.equal(r.wait(timeout=60), mpx.upscale("ravu-lite-ar-r3", f))
+        assert srv.launches - before == 1
+    finally:
+        srv.close()
+
