@@ -59,6 +59,22 @@ def timed(fn, steps, warmup, torch):
     return [s.elapsed_time(e) for s, e in zip(starts, ends)], wall
 
 
+def region_ms(fn, steps, warmup, torch):
+    """Average GPU time per call of fn() over `steps` back-to-back calls: one event pair on
+    the current stream around the whole timed region, divided by the call count (per-call
+    event pairs would also count the host's submission gaps)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / steps
+
+
 def pct(xs, q):
     xs = sorted(xs)
     return xs[min(len(xs) - 1, int(round(q / 100.0 * (len(xs) - 1))))]
@@ -249,8 +265,7 @@ def dominant_kernel(ctx, steps, torch):
     HBM traffic per launch comes from the hardware-counter summary committed beside the
     kernel-trace profiles (FETCH_SIZE + WRITE_SIZE of this launch, calibrated on a plain
     copy), when present."""
-    ktimes, _ = timed(_double_x(ctx, torch), steps, 5, torch)
-    kms = sum(ktimes) / len(ktimes)
+    kms = region_ms(_double_x(ctx, torch), max(steps, 20), 5, torch)
     flop = nnedi3_flop_per_px(HEADLINE) * W * 2 * H
     achieved = flop / (kms * 1e-3) / 1e12
     traffic = None
