@@ -111,13 +111,14 @@ struct Geo {
 template <int WH, int AXIS>
 using NTile = HaloTile<1, Geo<WH, AXIS>::TH, Geo<WH, AXIS>::TW, 256>;
 
-// mean, 1/std (0 below the variance threshold) and std*... of a K-sample window held in
-// registers; the window is replaced by its centred, normalised form.
-template <int K>
-__device__ __forceinline__ void normalise(float* x, float& m, float& sd) {
-  float sum = 0.f;
+// mean, 1/std (0 below the variance threshold) and std of a K-sample window held in
+// registers; the window is replaced by its centred form, and by its centred, normalised
+// form when SCALE (otherwise the caller applies 1/std, e.g. inside its 16-bit conversion).
+template <int K, bool SCALE = true>
+__device__ __forceinline__ void normalise(float* x, float& m, float& sd, float& is) {
+  float sum = x[0];
 #pragma unroll
-  for (int k = 0; k < K; ++k) sum += x[k];
+  for (int k = 1; k < K; ++k) sum += x[k];
   m = sum * (1.0f / static_cast<float>(K));
   float ss = 0.f;
 #pragma unroll
@@ -127,10 +128,12 @@ __device__ __forceinline__ void normalise(float* x, float& m, float& sd) {
   }
   const float v = ss * (1.0f / static_cast<float>(K));
   // hardware reciprocal square root (within 1 ulp); the threshold test is exact
-  const float is = v >= kVarEps ? __builtin_amdgcn_rsqf(v) : 0.f;
+  is = v >= kVarEps ? __builtin_amdgcn_rsqf(v) : 0.f;
   sd = v * is;
+  if constexpr (SCALE) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) x[k] *= is;
+    for (int k = 0; k < K; ++k) x[k] *= is;
+  }
 }
 
 template <int AXIS>
@@ -179,7 +182,8 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
 #pragma unroll
     for (int b = 0; b < WH; ++b) xv[a * WH + b] = tile[G::at(tx, ty, a, b)];
   float m, sd;
-  normalise<K>(xv, m, sd);
+  float is;
+  normalise<K>(xv, m, sd, is);
   float wsum = 0.f, vsum = 0.f;
   for (int n = 0; n < nns; ++n) {
     const float* w = wt + n * (2 * K + 2);
@@ -233,6 +237,9 @@ struct Mma {
   static constexpr int KP = 32 * KC;
   static constexpr int XS = KP + 8;  // LDS row stride (elements; padding vs bank conflicts)
   static constexpr int LO_ROWS = TERMS >= 3;
+  // one FP16 product: the window arrives centred but unscaled and 1/std is applied inside
+  // the conversion (one fused multiply-convert per value, v_fma_mix)
+  static constexpr bool FUSED_SCALE = FORM == F_F16 && TERMS == 1;
   struct W {
     typename O::V8 hi[KC], lo[KC];
   };
@@ -253,11 +260,12 @@ struct Mma {
         w.lo[c][j] = (TERMS >= 3 && in) ? L[(static_cast<int64_t>(n) * 2 + mtx) * K + k] : zero;
       }
   }
-  // one pixel's normalised window (scaled by xscale) -> its LDS row(s)
-  __device__ static void write_row(T* rh, T* rl, const float* xv, float xscale, bool scaled) {
+  // one pixel's normalised window (scaled by xscale) -> its LDS row(s); with FUSED_SCALE
+  // the window is centred only and is = 1/std
+  __device__ static void write_row(T* rh, T* rl, const float* xv, float xscale, bool scaled, float is) {
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
-      const float v = k < K ? (scaled ? xv[k] * xscale : xv[k]) : 0.f;
+      const float v = k < K ? (FUSED_SCALE ? fmaf(xv[k], is, 0.0f) : scaled ? xv[k] * xscale : xv[k]) : 0.f;
       const T h = O::cvt(v);
       rh[k] = h;
       if (TERMS >= 3) rl[k] = O::cvt(v - O::back(h));
@@ -313,7 +321,9 @@ struct Mma<F_FP8, K, TERMS> {
     w.a = int8v{lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
     w.sc = sc[n * 2 + mtx];
   }
-  __device__ static void write_row(T* rh, T* /*rl*/, const float* xv, float /*xscale*/, bool /*scaled*/) {
+  static constexpr bool FUSED_SCALE = false;
+  __device__ static void write_row(T* rh, T* /*rl*/, const float* xv, float /*xscale*/, bool /*scaled*/,
+                                   float /*is*/) {
     constexpr float XSC = static_cast<float>(1 << kFp8XExp);
     uint32_t hv[(K + 3) / 4], lv[(K + 3) / 4];
 #pragma unroll
@@ -408,8 +418,27 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
       b2[t][i] = bias[2 * nn + 1];
     }
   }
-  // accumulator scale: exp2 argument acc*inv_acc, and 1 + |t| -> acc_one + |acc|
-  const float acc_one = SCALED ? 1.0f / inv_acc : 1.0f;
+  // Softmax-weighted elliott sum with one reciprocal per 8 neurons (4 when a wave holds
+  // one neuron tile): sum_i e_i t_i / p_i with p_i = 1 + |t_i| is carried as a fraction
+  // N / D, combined pairwise ((N1 D2 + N2 D1) / (D1 D2)).  Bounds: with the window
+  // normalised (sum x' = 0, sum x'^2 = K), |s| <= sqrt(K) |w| + |b| (Cauchy-Schwarz), at
+  // most 63 for the elliott half and 25.2 for the exp2 half over every shipped network,
+  // so D <= 64^8 = 2^48 and N <= 8 * 2^26 * 64^8 = 2^77: far inside FP32.
+  // For SCALED forms the accumulators carry ACC = s * acc_scale: exp2 takes acc*inv_acc,
+  // p = 1 + |acc| * inv_acc, and N (hence vs) carries acc_scale, removed at the end.
+  auto quad_frac = [&](const float4v& a1, const float4v& a2, float& ws, float& n, float& d) {
+    float e[4], q[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      e[i] = __builtin_amdgcn_exp2f(SCALED ? a1[i] * inv_acc : a1[i]);
+      ws += e[i];
+      q[i] = SCALED ? fmaf(fabsf(a2[i]), inv_acc, 1.0f) : 1.0f + fabsf(a2[i]);
+    }
+    const float n01 = fmaf(e[0] * a2[0], q[1], e[1] * a2[1] * q[0]), d01 = q[0] * q[1];
+    const float n23 = fmaf(e[2] * a2[2], q[3], e[3] * a2[3] * q[2]), d23 = q[2] * q[3];
+    n = fmaf(n01, d23, n23 * d01);
+    d = d01 * d23;
+  };
 
   // softmax-weighted sums of one pixel group over this wave's neuron tiles; on return
   // every lane holds the sums of pixel (g*16 + row)
@@ -418,19 +447,23 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     M::load_b(bx, xhi, xlo, g * 16 + row, quad);  // B column of this lane: pixel g*16 + row
     ws = 0.f;
     vs = 0.f;
+    float n = 0.f, d = 1.f;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
       const float4v a1 = M::contract(w1[t], bx, b1[t]);
       const float4v a2 = M::contract(w2[t], bx, b2[t]);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float e = __builtin_amdgcn_exp2f(SCALED ? a1[i] * inv_acc : a1[i]);
-        // t/(1+|t|) with t = a2/acc_one  ==  a2/(acc_one + |a2|)
-        const float q = a2[i] * __builtin_amdgcn_rcpf(acc_one + fabsf(a2[i]));
-        ws += e;
-        vs = fmaf(e, q, vs);
+      float nq, dq;
+      quad_frac(a1, a2, ws, nq, dq);
+      if (t % 2 == 0) {
+        n = nq;
+        d = dq;
+      } else {
+        n = fmaf(n, dq, nq * d);
+        d *= dq;
       }
+      if (t % 2 == 1 || t == TPW - 1) vs = fmaf(n, __builtin_amdgcn_rcpf(d), vs);
     }
+    if constexpr (SCALED) vs *= inv_acc;
     // lanes l, l^16, l^32, l^48 hold the same pixel
     ws += __shfl_xor(ws, 16);
     vs += __shfl_xor(vs, 16);
@@ -469,8 +502,9 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
 #pragma unroll
         for (int b = 0; b < WH; ++b) xv[a * WH + b] = tl[G::at(tx, ty, a, b)];
       orig = tl[G::centre(tx, ty)];
-      normalise<K>(xv, m, sd);
-      M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED);
+      float is;
+      normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
+      M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED, is);
     }
 
     float wsum = 0.f, vsum = 0.f;
