@@ -226,6 +226,13 @@ __device__ __forceinline__ void store_pair(void* p, int64_t idx, int dt, float v
   }
 }
 
+// two FP16 values packed in one dword (low half first)
+__device__ __forceinline__ __half2 bits_h2(uint32_t v) {
+  __half2 h;
+  __builtin_memcpy(&h, &v, sizeof(h));
+  return h;
+}
+
 __device__ __forceinline__ float4 load_lut4(const __half* lut, int64_t texel) {
   const __half2* p = reinterpret_cast<const __half2*>(lut + 4 * texel);
   float2 lo = __half22float2(p[0]);

@@ -78,28 +78,49 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   for (int c = 0; c < P; ++c)
 #pragma unroll
     for (int k = 0; k < 4; ++k) r0[c][k] = r1[c][k] = 0.f;
+  // One step per tap pair (two LUT texels), software-pipelined: the next step's texels
+  // are fetched (as raw FP16, converted inside the multiply-adds) while this step
+  // computes, and each step's sums complete before the next step's samples are read, so
+  // only one step's operands are live (r3 luma: 46 VGPRs instead of 134).
+  const uint2* lraw = reinterpret_cast<const uint2*>(lrow);
+  uint2 t0 = lraw[0], t1 = lraw[1];
 #pragma unroll
   for (int i = 0; i <= C; ++i) {
-    const float4 w0 = load_lut4(lrow, 2 * i), w1 = load_lut4(lrow, 2 * i + 1);
-    const float a0[4] = {w0.x, w0.y, w0.z, w0.w}, a1[4] = {w1.x, w1.y, w1.z, w1.w};
+    __builtin_amdgcn_sched_barrier(0);
+    uint2 n0 = t0, n1 = t1;
+    if (i < C) {
+      n0 = lraw[2 * i + 2];
+      n1 = lraw[2 * i + 3];
+    }
+    const __half2 h00 = bits_h2(t0.x), h01 = bits_h2(t0.y), h10 = bits_h2(t1.x), h11 = bits_h2(t1.y);
+    const float a0[4] = {__low2float(h00), __high2float(h00), __low2float(h01), __high2float(h01)};
+    const float a1[4] = {__low2float(h10), __high2float(h10), __low2float(h11), __high2float(h11)};
 #pragma unroll
     for (int c = 0; c < P; ++c) {
       const float li = tile[c * PN + off(i)];
       if (i < C) {
         const float lj = tile[c * PN + off(N * N - 1 - i)];
+        // res0 += l_i*w0 + l_j*w1.wzyx, res1 += l_i*w1 + l_j*w0.wzyx, as four fused
+        // multiply-adds (the LUT operand converts inside them, v_fma_mix)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          r0[c][k] += li * a0[k] + lj * a1[3 - k];   // res0 += l_i*w0 + l_j*w1.wzyx
-          r1[c][k] += li * a1[k] + lj * a0[3 - k];   // res1 += l_i*w1 + l_j*w0.wzyx
+          r0[c][k] = fmaf(lj, a1[3 - k], fmaf(li, a0[k], r0[c][k]));
+          r1[c][k] = fmaf(lj, a0[3 - k], fmaf(li, a1[k], r1[c][k]));
         }
       } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          r0[c][k] += li * a0[k];
-          r1[c][k] += li * a1[k];
+          r0[c][k] = fmaf(li, a0[k], r0[c][k]);
+          r1[c][k] = fmaf(li, a1[k], r1[c][k]);
         }
       }
     }
+#pragma unroll
+    for (int c = 0; c < P; ++c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(r0[c][k]), "+v"(r1[c][k]));
+    t0 = n0;
+    t1 = n1;
   }
   const int esz_out = elem_bytes(io.out_dt);
   char* ob = static_cast<char*>(frame_out(io, f));
@@ -160,6 +181,13 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
 #pragma unroll
         for (int c = 0; c < P; ++c) res[c] += (s(c, p) + s(c, N * N - 1 - p)) * a[j];
       }
+    }
+    // ... and this texel's sums complete before the next texel's loads: without this the
+    // multiply-adds sink below every load and all N*N*P samples are live at once (r4 yuv:
+    // 216 VGPRs, 2 waves per SIMD; with it 128, 4 waves)
+    if constexpr (N * N * P >= 96) {
+#pragma unroll
+      for (int c = 0; c < P; ++c) asm volatile("" : "+v"(res[c]));
     }
   }
 #pragma unroll

@@ -62,21 +62,29 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   const __half* lrow = lut + static_cast<int64_t>(row) * NTEX * 4;
   float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
   float4 hi = res, lo = res, hi2 = res, lo2 = res;
+  // One step per LUT texel, software-pipelined: the next texel is fetched (raw FP16,
+  // converted inside the multiply-adds) while this one computes, and each step's sums
+  // complete before the next step starts, so one step's operands are live at a time.
+  const uint2* lraw = reinterpret_cast<const uint2*>(lrow);
+  uint2 tc = lraw[0];
 #pragma unroll
   for (int i = 0; i <= C; ++i) {
-    const float4 w = load_lut4(lrow, i);
+    __builtin_amdgcn_sched_barrier(0);
+    uint2 tn = tc;
+    if (i < C) tn = lraw[i + 1];
+    const __half2 h0 = bits_h2(tc.x), h1 = bits_h2(tc.y);
+    const float4 w = make_float4(__low2float(h0), __high2float(h0), __low2float(h1), __high2float(h1));
     const float li = s[i];
-    if (i < C) {
+    res.x = fmaf(li, w.x, res.x);
+    res.y = fmaf(li, w.y, res.y);
+    res.z = fmaf(li, w.z, res.z);
+    res.w = fmaf(li, w.w, res.w);
+    if (i < C) {  // the mirrored sample takes the texel reversed (w.wzyx)
       const float lj = s[N * N - 1 - i];
-      res.x += li * w.x + lj * w.w;
-      res.y += li * w.y + lj * w.z;
-      res.z += li * w.z + lj * w.y;
-      res.w += li * w.w + lj * w.x;
-    } else {
-      res.x += li * w.x;
-      res.y += li * w.y;
-      res.z += li * w.z;
-      res.w += li * w.w;
+      res.x = fmaf(lj, w.w, res.x);
+      res.y = fmaf(lj, w.z, res.y);
+      res.z = fmaf(lj, w.y, res.z);
+      res.w = fmaf(lj, w.x, res.w);
     }
     if (AR) {
       const int X = i / N - HALO, Y = i % N - HALO;
@@ -104,8 +112,14 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
           hi2.x += q33 * wg.w; hi2.y += q33 * wg.z; hi2.z += q33 * wg.y; hi2.w += q33 * wg.x;
           lo2.x += n33 * wg.w; lo2.y += n33 * wg.z; lo2.z += n33 * wg.y; lo2.w += n33 * wg.x;
         }
+        asm volatile("" : "+v"(hi.x), "+v"(hi.y), "+v"(hi.z), "+v"(hi.w), "+v"(lo.x), "+v"(lo.y), "+v"(lo.z),
+                     "+v"(lo.w));
+        asm volatile("" : "+v"(hi2.x), "+v"(hi2.y), "+v"(hi2.z), "+v"(hi2.w), "+v"(lo2.x), "+v"(lo2.y),
+                     "+v"(lo2.z), "+v"(lo2.w));
       }
     }
+    asm volatile("" : "+v"(res.x), "+v"(res.y), "+v"(res.z), "+v"(res.w));
+    tc = tn;
   }
   float r4[4] = {res.x, res.y, res.z, res.w};
   if (AR) {
