@@ -32,10 +32,29 @@ __device__ __forceinline__ void stage_planes(float* tile, const void* base, int 
   h.commit(tile, dt, in_max);
 }
 
-// Hash-plane value from P staged planes at LDS offset o (plane stride n).
+// Planes staged per tile: the P input planes, plus for rgb a fourth holding the BT.709
+// luma the hash reads (computed once per staged sample, not once per window use).
+template <int CH>
+constexpr int staged_planes() { return CH == 0 ? 1 : (CH == 1 ? 3 : 4); }
+
+template <int CH>
+__device__ __forceinline__ float luma_of(const float* t, int o, int n) {
+  return 0.2126f * t[o] + 0.7152f * t[n + o] + 0.0722f * t[2 * n + o];
+}
+
+// rgb: fill plane 3 of a staged tile (n samples per plane) with the luma of planes 0-2;
+// the caller synchronises the block before and after
+template <int CH>
+__device__ __forceinline__ void stage_luma(float* t, int n) {
+  if constexpr (CH == 2) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) t[3 * n + i] = luma_of<CH>(t, i, n);
+  }
+}
+
+// Hash-plane value from the staged planes at LDS offset o (plane stride n).
 template <int CH>
 __device__ __forceinline__ float luma_f(const float* t, int o, int n) {
-  if (CH == 2) return 0.2126f * t[o] + 0.7152f * t[n + o] + 0.0722f * t[2 * n + o];
+  if (CH == 2) return t[3 * n + o];
   return t[o];
 }
 template <int CH>
@@ -57,13 +76,17 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   constexpr int PW = RT_X + 2 * HALO, PH = RT_Y + 2 * HALO, PN = PW * PH;
   constexpr int C = (N * N - 1) / 2;
   constexpr int NTEX = 2 * (C + 1);
-  __shared__ float tile[P * PN];
+  __shared__ float tile[staged_planes<CH>() * PN];
   const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
   stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - HALO, y0 - HALO);
   __syncthreads();
+  if constexpr (CH == 2) {
+    stage_luma<CH>(tile, PN);
+    __syncthreads();
+  }
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
@@ -168,10 +191,10 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
   for (int c = 0; c < P; ++c) res[c] = 0.f;
 #pragma unroll
   for (int t = 0; t < RavuLut<N>::NTEX; ++t) {
-    // large windows (e.g. r4 with three planes: 192 samples): one LUT texel's taps at a
-    // time, so the loads of all samples are not hoisted together (register pressure,
-    // hence occupancy: measured -7 % time on ravu-r4-yuv; small windows lose by it)
-    if constexpr (N * N * P >= 96) __builtin_amdgcn_sched_barrier(0);
+    // large windows (r4: 64 samples per plane): one LUT texel's taps at a time, so the
+    // loads of all samples are not hoisted together (register pressure, hence occupancy;
+    // small windows lose by it)
+    if constexpr (N * N * P >= 64) __builtin_amdgcn_sched_barrier(0);
     const float4 w = load_lut4(lrow, t);
     const float a[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
@@ -183,9 +206,9 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
       }
     }
     // ... and this texel's sums complete before the next texel's loads: without this the
-    // multiply-adds sink below every load and all N*N*P samples are live at once (r4 yuv:
-    // 216 VGPRs, 2 waves per SIMD; with it 128, 4 waves)
-    if constexpr (N * N * P >= 96) {
+    // multiply-adds sink below every load and all N*N*P samples are live at once (r4 yuv
+    // lattice: 216 VGPRs, 2 waves per SIMD; with it 128, 4 waves)
+    if constexpr (N * N * P >= 64) {
 #pragma unroll
       for (int c = 0; c < P; ++c) asm volatile("" : "+v"(res[c]));
     }
@@ -201,7 +224,7 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
   constexpr int PW = RT_X + N - 1, PH = RT_Y + N - 1, PN = PW * PH;
-  __shared__ float tile[P * PN];
+  __shared__ float tile[staged_planes<CH>() * PN];
   const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
@@ -209,6 +232,10 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
                   y0 - (R - 1));
   __syncthreads();
+  if constexpr (CH == 2) {
+    stage_luma<CH>(tile, PN);
+    __syncthreads();
+  }
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
@@ -245,8 +272,8 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
   constexpr int PW = RT_X + 2 * R + 1, PH = RT_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
-  __shared__ float hk[P * PN];   // input (HOOKED)
-  __shared__ float i11[P * PN];  // ravu_int11
+  __shared__ float hk[staged_planes<CH>() * PN];   // input (HOOKED)
+  __shared__ float i11[staged_planes<CH>() * PN];  // ravu_int11
   const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
@@ -260,6 +287,11 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
     b.commit(i11, DT_F32, 1.f);
   }
   __syncthreads();
+  if constexpr (CH == 2) {
+    stage_luma<CH>(hk, PN);
+    stage_luma<CH>(i11, PN);
+    __syncthreads();
+  }
   const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;

@@ -5,7 +5,8 @@
 // Per output pixel o: pos = (o + 1/2) * in/out (OFFSET ALIGN); subpix f = fract(pos - 1/2);
 // base = pos - f.  The window is the 2r x 2r input block at offsets -(r-1)..r around base.
 // The hash depends on the base pixel only, so a block first hashes every base pixel its
-// 32 x 8 output tile touches (into LDS), then each output pixel reads its base's bucket.
+// 32 x 8 output tile touches (into LDS), then each output pixel reads its base's bucket;
+// AR variants also tabulate the per-sample powers of the anti-ringing weights there.
 // Base and phase are computed in exact integer arithmetic:
 //   pos - 1/2 = ((2o+1)*in - out) / (2*out)  ->  base = floor, f = remainder / (2*out).
 // LUT (FILTER LINEAR): group g of 4 taps at texel x = 9g + 8f, row y = 9*bucket + 8f_y
@@ -93,12 +94,18 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   constexpr int PW = ZB_X + N - 1, PH = ZB_Y + N - 1, PN = PW * PH;
   __shared__ float tile[P * PN];
   __shared__ int bucket[ZB_Y * ZB_X];
+  // AR: per staged sample (l + 0.1)^32, (1.1 - l)^32 and the ^33 terms, computed once per
+  // sample instead of once per output pixel that reads it (about 25 at 2x)
+  __shared__ float4 pw[AR ? P * PN : 1];
 
   const BlockXYZ bk3 = block_xyz((out_w + ZT_X - 1) / ZT_X, (out_h + ZT_Y - 1) / ZT_Y, io.batch);
   const int f = bk3.z;
   const int ox0 = bk3.x * ZT_X, oy0 = bk3.y * ZT_Y;
   const int bx0 = phase_of(ox0, io.W, out_w).base;
   const int by0 = phase_of(oy0, io.H, out_h).base;
+  // base pixels this tile's (in-range) outputs touch: nbx x nby <= ZB_X x ZB_Y (in < out)
+  const int nbx = phase_of(min(ox0 + ZT_X, out_w) - 1, io.W, out_w).base - bx0 + 1;
+  const int nby = phase_of(min(oy0 + ZT_Y, out_h) - 1, io.H, out_h).base - by0 + 1;
   const void* in = frame_in(io, f);
   {
     HaloTile<P, PH, PW, 256> h;
@@ -106,24 +113,34 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
     h.commit(tile, io.in_dt, io.in_max);
   }
   __syncthreads();
-  // hash every base pixel of the tile (unclamped LUT row: bucket 9-row blocks are clamped
-  // per texel by the bilinear fetch, as the sampler does)
-  for (int i = threadIdx.x; i < ZB_Y * ZB_X; i += blockDim.x) {
-    const int by = i / ZB_X, bx = i - by * ZB_X;
+  // hash every base pixel the tile's outputs use (unclamped LUT row: bucket 9-row blocks
+  // are clamped per texel by the bilinear fetch, as the sampler does)
+  for (int i = threadIdx.x; i < nbx * nby; i += blockDim.x) {
+    const int by = i / nbx, bx = i - by * nbx;
     auto off = [&](int s) { return (by + s % N) * PW + bx + s / N; };
     const Bucket bk = ravu_hash<N, true, RULE_LITE>([&](int s) { return zluma_f<CH>(tile, off(s), PN); },
                                                     [&](int s) { return zluma_d<CH>(tile, off(s), PN); });
-    bucket[i] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+    bucket[by * ZB_X + bx] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+  }
+  if constexpr (AR) {
+    for (int i = threadIdx.x; i < P * PN; i += blockDim.x) {
+      const float l = tile[i];
+      const float p0 = 0.1f + l, m0 = 1.1f - l;
+      float p32 = p0, m32 = m0;
+#pragma unroll
+      for (int e = 0; e < 5; ++e) { p32 *= p32; m32 *= m32; }
+      pw[i] = make_float4(p32, m32, p32 * p0, m32 * m0);
+    }
   }
   __syncthreads();
 
   const int ox = ox0 + threadIdx.x % ZT_X, oy = oy0 + threadIdx.x / ZT_X;
   if (ox >= out_w || oy >= out_h) return;
   const Phase phx = phase_of(ox, io.W, out_w), phy = phase_of(oy, io.H, out_h);
-  const int lbx = phx.base - bx0, lby = phy.base - by0;  // in [0, ZB) by construction
+  const int lbx = phx.base - bx0, lby = phy.base - by0;  // in [0, nbx) x [0, nby) by construction
   const int row = bucket[lby * ZB_X + lbx];
   if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(oy) * out_w + ox] = row;
-  auto s = [&](int c, int i) { return tile[c * PN + (lby + i % N) * PW + lbx + i / N]; };
+  auto si = [&](int c, int i) { return c * PN + (lby + i % N) * PW + lbx + i / N; };
   const float txf = 8.f * phx.f, txi = 8.f * (1.f - phx.f);
   const float tyf = 8.f * phy.f, tyi = 8.f * (1.f - phy.f);
   float res[P], hi[P], lo[P], hi2[P], lo2[P];
@@ -131,6 +148,8 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   for (int c = 0; c < P; ++c) res[c] = hi[c] = lo[c] = hi2[c] = lo2[c] = 0.f;
 #pragma unroll
   for (int g = 0; g < GROUPS; ++g) {
+    // one group's operands live at a time (its sums complete before the next group)
+    __builtin_amdgcn_sched_barrier(0);
     const float4 wf4 = lut_bilinear(lut, WIDTH, ROWS, g, txf, row, tyf);
     const float4 wi4 = lut_bilinear(lut, WIDTH, ROWS, g, txi, row, tyi);
     const float wf[4] = {wf4.x, wf4.y, wf4.z, wf4.w}, wi[4] = {wi4.x, wi4.y, wi4.z, wi4.w};
@@ -148,21 +167,23 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
         const int q = N * N - 1 - p;
 #pragma unroll
         for (int c = 0; c < P; ++c) {
-          const float lp = s(c, p), lq = s(c, q);
+          const float lp = tile[si(c, p)], lq = tile[si(c, q)];
           res[c] += lp * wf[k] + lq * wi[k];
           if (AR) {
             // ravu-zoom-ar-r2-rgb.hook:152-222: (0.1+l)^32, (1.1-l)^32 and the ^33 terms
-            float p0 = 0.1f + lp, m0 = 1.1f - lp, p1 = 0.1f + lq, m1 = 1.1f - lq;
-            float p32 = p0, m32 = m0, q32 = p1, n32 = m1;
-#pragma unroll
-            for (int e = 0; e < 5; ++e) { p32 *= p32; m32 *= m32; q32 *= q32; n32 *= n32; }
-            hi[c] += p32 * af[k] + q32 * ai[k];
-            lo[c] += m32 * af[k] + n32 * ai[k];
-            hi2[c] += p32 * p0 * af[k] + q32 * p1 * ai[k];
-            lo2[c] += m32 * m0 * af[k] + n32 * m1 * ai[k];
+            const float4 a = pw[si(c, p)], b = pw[si(c, q)];
+            hi[c] += a.x * af[k] + b.x * ai[k];
+            lo[c] += a.y * af[k] + b.y * ai[k];
+            hi2[c] += a.z * af[k] + b.z * ai[k];
+            lo2[c] += a.w * af[k] + b.w * ai[k];
           }
         }
       }
+    }
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      asm volatile("" : "+v"(res[c]));
+      if (AR) asm volatile("" : "+v"(hi[c]), "+v"(lo[c]), "+v"(hi2[c]), "+v"(lo2[c]));
     }
   }
   const int esz_out = elem_bytes(io.out_dt);
