@@ -145,13 +145,21 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
 /* Autotune the plan's launch shape on the caller's buffers: runs the plan on `io`
  * (warm-up + `iters` timed runs per candidate, HIP events on hip_stream, synchronising)
  * over the candidate shapes and keeps the fastest in the plan.  NNEDI3 MFMA plans: the
- * persistent grid (blocks per CU) and, for nns = 128, the neuron split across waves.
+ * persistent grid (blocks per CU) and, for nns = 64 and 128, the neuron split across waves.
  * Other plans have no tunable shape: returns MPX_OK with *grid_per_cu = 0.  Writes the
  * output buffers.  Outputs (may be NULL): the chosen blocks per CU (0 = occupancy
  * maximum) and split, and the best time per run in microseconds.  Replaces the fixed
  * workgroup shape of the compute shaders (e.g. compute/ravu-3x-r3.hook:22 `//!COMPUTE`). */
 int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters, int* grid_per_cu, int* split,
                   float* best_us);
+
+/* Set the plan's launch shape directly, e.g. one found by mpx_plan_tune in an earlier
+ * process (the tuner's result can be cached per GPU model and frame size).  grid_per_cu:
+ * 0 (occupancy maximum) or 1..64 blocks per CU (capped at the occupancy maximum); split:
+ * 0, or 1 for NNEDI3 MFMA plans with nns = 64 or 128.  Plans without a tunable shape
+ * accept only (0, 0).  Enqueues nothing.  No counterpart in the reference (its compute
+ * shaders fix the workgroup shape, e.g. compute/ravu-3x-r3.hook:22). */
+int mpx_plan_set_shape(mpx_plan* plan, int grid_per_cu, int split);
 
 /* ---- the scaling pass that consumes a prescaler's OFFSET ---------------------------- */
 #define MPX_FILTER_BILINEAR 0

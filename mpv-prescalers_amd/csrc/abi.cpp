@@ -500,6 +500,22 @@ int mpx_resample(const mpx_io* io, int in_w, int in_h, int in_type, int bits, in
   return MPX_OK;
 }
 
+int mpx_plan_set_shape(mpx_plan* plan, int grid_per_cu, int split) {
+  if (!plan) return fail(MPX_ERR_ARG, "plan is NULL");
+  const bool nn = plan->v->family == MPX_FAMILY_NNEDI3 && plan->precision != MPX_PREC_FP32;
+  if (!nn) {
+    if (grid_per_cu != 0 || split != 0) return fail(MPX_ERR_ARG, "this plan has no tunable launch shape");
+    return MPX_OK;
+  }
+  if (grid_per_cu < 0 || grid_per_cu > 64) return fail(MPX_ERR_ARG, "grid_per_cu outside 0..64");
+  const bool can_split = plan->v->nns == 64 || plan->v->nns == 128;
+  if (split < 0 || split > 1 || (split && !can_split))
+    return fail(MPX_ERR_ARG, "split must be 0, or 1 for nns 64 / 128");
+  plan->nw.grid_per_cu = grid_per_cu;
+  plan->nw.split = split;
+  return MPX_OK;
+}
+
 int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters, int* grid_per_cu, int* split,
                   float* best_us) {
   if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
@@ -519,7 +535,7 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
     return fail(MPX_ERR_HIP, "hipEventCreate failed");
   }
   const int grids[] = {0, 1, 2, 3, 4, 6, 8};
-  const int nsplit = nn && plan->v->nns == 128 ? 2 : 1;
+  const int nsplit = nn && (plan->v->nns == 64 || plan->v->nns == 128) ? 2 : 1;
   const mpx::NnediWeights saved = plan->nw;
   auto set_shape = [&](int g, int sp) {
     plan->nw.grid_per_cu = g;

@@ -26,8 +26,9 @@ struct NnediWeights {
   const float* bias = nullptr;
   const int* wexp = nullptr;
   // launch shape (set by the autotuner; 0 = defaults): persistent blocks per CU (capped at
-  // the kernel's occupancy), and for nns = 128 the neuron split (0: 2 slices x 4 tiles,
-  // 1: 4 slices x 2 tiles per wave)
+  // the kernel's occupancy), and for nns = 64 / 128 the neuron split (nns 64 -- 0: every
+  // wave holds all 4 tiles, 1: 2 slices x 2 tiles; nns 128 -- 0: 2 slices x 4 tiles, 1: 4
+  // slices x 2 tiles per wave)
   int grid_per_cu = 0;
   int split = 0;
 };

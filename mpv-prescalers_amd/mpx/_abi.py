@@ -24,7 +24,8 @@ CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 # every exported symbol declared in include/mpx.h
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
-           "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_resample", "mpx_last_error")
+           "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_plan_set_shape", "mpx_resample",
+           "mpx_last_error")
 MAX_FRAMES = 16
 
 
@@ -80,6 +81,7 @@ def _load():
                                        ctypes.POINTER(i32), ctypes.POINTER(i64), ctypes.POINTER(i64), vp]),
         "mpx_plan_tune": (i32, [vp, ctypes.POINTER(MpxIo), vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                 ctypes.POINTER(ctypes.c_float)]),
+        "mpx_plan_set_shape": (i32, [vp, i32, i32]),
         "mpx_resample": (i32, [ctypes.POINTER(MpxIo), i32, i32, i32, i32, i32, i32, i32, i32, i32,
                                ctypes.c_float, ctypes.c_float, vp]),
         "mpx_last_error": (ctypes.c_char_p, []),

@@ -95,6 +95,11 @@ class Plan:
                                 ctypes.byref(g), ctypes.byref(sp), ctypes.byref(us)))
         return {"grid_per_cu": g.value, "split": sp.value, "us": us.value}
 
+    def set_shape(self, grid_per_cu=0, split=0):
+        """Set the launch shape directly (e.g. a cached ``tune`` result): blocks per CU
+        (0 = occupancy maximum) and, for NNEDI3 nns 64 / 128, the neuron split."""
+        check(LIB.mpx_plan_set_shape(self._h, int(grid_per_cu), int(split)))
+
     def _io(self, x, out, buckets):
         planes = self.info["planes"]
         if x.dtype != self.in_dtype:

@@ -38,6 +38,23 @@ def test_nnedi3_parity_small(nns, wh, prec):
     assert np.array_equal(got[0::2, 0::2], img)
 
 
+@pytest.mark.parametrize("nns", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("wh", [4, 6])
+def test_nnedi3_fp16_bound_10bit(nns, wh):
+    """The FP16 form's 2^-10 bound on 10-bit frames, where the window's FP16 rounding
+    weighs most (nns 128 runs the split-window form to stay inside it)."""
+    import mpx
+    from oracle.nnedi3 import nnedi3
+    worst = 0.0
+    for seed in (1000, 1001, 1002):
+        img = _frame(96, 64, seed=seed, bits=10)
+        got = mpx.upscale("nnedi3-nns%d-win8x%d" % (nns, wh), torch.from_numpy(img).cuda(), precision="fp16")
+        f = compare("nnedi3-nns%d-win8x%d fp16 96x64 10-bit seed %d" % (nns, wh, seed), got.cpu().numpy(),
+                    nnedi3(img.astype(np.float64), nns, wh))
+        worst = max(worst, f["max_abs"])
+    assert worst <= TOL_MFMA
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16x3"])
 def test_nnedi3_nns32_1080p_fp16_input(prec):
     """The 1920x1080 -> 3840x2160 workload: fp16 input plane, fp16 output."""

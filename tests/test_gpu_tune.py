@@ -9,7 +9,7 @@ from parity import compare
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("nns,wh", [(32, 4), (128, 6)])
+@pytest.mark.parametrize("nns,wh", [(32, 4), (64, 6), (128, 6)])
 def test_tune_nnedi3(nns, wh):
     import mpx
     from mpx.frames import make_plane
@@ -19,7 +19,7 @@ def test_tune_nnedi3(nns, wh):
     plan = mpx.Context(0).plan("nnedi3-nns%d-win8x%d" % (nns, wh), 160, 96, precision="fp16")
     res = plan.tune(x, iters=3)
     assert res["grid_per_cu"] in (0, 1, 2, 3, 4, 6, 8) and res["us"] > 0
-    assert res["split"] in ((0, 1) if nns == 128 else (0,))
+    assert res["split"] in ((0, 1) if nns in (64, 128) else (0,))
     f = compare("nnedi3-nns%d-win8x%d fp16 tuned %s" % (nns, wh, res), plan.run(x).cpu().numpy(),
                 nnedi3(img.astype(np.float64), nns, wh))
     assert f["max_abs"] <= 2.0 ** -10
@@ -32,3 +32,35 @@ def test_tune_other_plans_untouched():
     before = plan.run(x).clone()
     assert plan.tune(x) == {"grid_per_cu": 0, "split": 0, "us": 0.0}
     assert torch.equal(plan.run(x), before)
+
+
+@pytest.mark.parametrize("nns,wh,split,grid", [(64, 6, 0, 0), (64, 6, 1, 0), (64, 4, 1, 2), (128, 6, 0, 3),
+                                               (128, 6, 1, 0)])
+def test_set_shape_nnedi3(nns, wh, split, grid):
+    """Every neuron split / grid the tuner can pick meets the parity bar when set directly."""
+    import mpx
+    from mpx.frames import make_plane
+    from oracle.nnedi3 import nnedi3
+    img = make_plane(136, 72, 11, bits=10).astype(np.float32)
+    x = torch.from_numpy(img).cuda()
+    plan = mpx.Context(0).plan("nnedi3-nns%d-win8x%d" % (nns, wh), 136, 72, precision="fp16")
+    plan.set_shape(grid, split)
+    f = compare("nnedi3-nns%d-win8x%d fp16 split %d grid %d" % (nns, wh, split, grid), plan.run(x).cpu().numpy(),
+                nnedi3(img.astype(np.float64), nns, wh))
+    assert f["max_abs"] <= 2.0 ** -10
+
+
+def test_set_shape_rejects():
+    import mpx
+    from mpx._abi import MpxError
+    ctx = mpx.Context(0)
+    nn32 = ctx.plan("nnedi3-nns32-win8x4", 64, 32, precision="fp16")
+    with pytest.raises(MpxError):
+        nn32.set_shape(0, 1)          # nns 32 has no neuron split
+    with pytest.raises(MpxError):
+        nn32.set_shape(65, 0)
+    nn32.set_shape(4, 0)
+    lite = ctx.plan("ravu-lite-r3", 64, 32)
+    lite.set_shape(0, 0)
+    with pytest.raises(MpxError):
+        lite.set_shape(2, 0)          # no tunable shape
