@@ -242,23 +242,37 @@ struct Mma {
   // inside the conversions (one fused multiply-convert per part, v_fma_mix: high =
   // f16(x * is), low = f16(x * is - high))
   static constexpr bool FUSED_SCALE = FORM == F_F16 && TERMS <= 2;
+  // FP16 forms with spare K slots (K = 48 in two 32-chunks): the bias rides in slots K and
+  // K+1 as FP16 high and low parts of b * 2^10 against window values 2^-10 (exact in
+  // FP16, |b| <= 4.4 over the shipped networks), so the accumulators start at zero and
+  // no bias registers are held (win8x6 nns64: 183 -> 151 VGPRs, one more wave per SIMD)
+  static constexpr bool BIAS_IN_K = FORM == F_F16 && TERMS <= 2 && K % 32 != 0;
+  static constexpr float kBiasX = 1.0f / 1024.0f;
   struct W {
     typename O::V8 hi[KC], lo[KC];
   };
   struct B {
     typename O::V8 h[KC], l[KC];
   };
-  __device__ static void load_w(W& w, const void* wh, const void* wl, const int* /*sc*/, int n, int mtx, int quad) {
+  __device__ static void load_w(W& w, const void* wh, const void* wl, const int* /*sc*/, const float* bias, int n,
+                                int mtx, int quad) {
     const T* H = static_cast<const T*>(wh);
     const T* L = static_cast<const T*>(wl);
     const T zero = static_cast<T>(0.0f);
+    T bh = zero, bl = zero;
+    if constexpr (BIAS_IN_K) {
+      const float b = bias[2 * n + mtx] * 1024.0f;
+      bh = O::cvt(b);
+      bl = O::cvt(b - O::back(bh));
+    }
 #pragma unroll
     for (int c = 0; c < KC; ++c)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int k = 32 * c + 8 * quad + j;
         const bool in = k < K;
-        w.hi[c][j] = in ? H[(static_cast<int64_t>(n) * 2 + mtx) * K + k] : zero;
+        w.hi[c][j] = in ? H[(static_cast<int64_t>(n) * 2 + mtx) * K + k]
+                        : (BIAS_IN_K && k == K ? bh : (BIAS_IN_K && k == K + 1 ? bl : zero));
         w.lo[c][j] = (TERMS >= 3 && in) ? L[(static_cast<int64_t>(n) * 2 + mtx) * K + k] : zero;
       }
   }
@@ -268,6 +282,11 @@ struct Mma {
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       if constexpr (FUSED_SCALE) {
+        if (BIAS_IN_K && (k == K || k == K + 1)) {  // the bias slots
+          rh[k] = O::cvt(kBiasX);
+          if (TERMS >= 2) rl[k] = O::cvt(0.0f);
+          continue;
+        }
         const float x = k < K ? xv[k] : 0.f;
         const T h = O::cvt(fmaf(x, is, 0.0f));
         rh[k] = h;
@@ -321,7 +340,9 @@ struct Mma<F_FP8, K, TERMS> {
   struct B {
     int8v b;
   };
-  __device__ static void load_w(W& w, const void* wh, const void* /*wl*/, const int* sc, int n, int mtx, int quad) {
+  static constexpr bool BIAS_IN_K = false;
+  __device__ static void load_w(W& w, const void* wh, const void* /*wl*/, const int* sc, const float* /*bias*/, int n,
+                                int mtx, int quad) {
     const int4* src = reinterpret_cast<const int4*>(static_cast<const uint8_t*>(wh) +
                                                     (static_cast<int64_t>(n) * 2 + mtx) * 128 + 32 * quad);
     const int4 lo = src[0], hi = src[1];
@@ -416,14 +437,15 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
 #pragma unroll
   for (int t = 0; t < TPW; ++t) {
     const int n = (nslice * TPW + t) * 16 + row;          // neuron of this lane's A row
-    M::load_w(w1[t], w_hi, w_lo, wsc, n, 0, quad);
-    M::load_w(w2[t], w_hi, w_lo, wsc, n, 1, quad);
-    // D rows (neurons) of this lane: 4*quad + i
+    M::load_w(w1[t], w_hi, w_lo, wsc, bias, n, 0, quad);
+    M::load_w(w2[t], w_hi, w_lo, wsc, bias, n, 1, quad);
+    // D rows (neurons) of this lane: 4*quad + i; the accumulators start at the biases
+    // unless the bias rides in the contraction's spare K slots
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int nn = (nslice * TPW + t) * 16 + 4 * quad + i;
-      b1[t][i] = bias[2 * nn];
-      b2[t][i] = bias[2 * nn + 1];
+      b1[t][i] = M::BIAS_IN_K ? 0.f : bias[2 * nn];
+      b2[t][i] = M::BIAS_IN_K ? 0.f : bias[2 * nn + 1];
     }
   }
   // Softmax-weighted elliott sum with one reciprocal per 8 neurons (4 when a wave holds
