@@ -1,6 +1,8 @@
 // Host-side launchers of the HIP kernels (one per kernel family), called by abi.cpp.
 #pragma once
 
+#include <atomic>
+
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -8,16 +10,17 @@
 
 namespace mpx {
 
-// Compute units of the current device (queried once; 256 on MI355X).
+// Compute units of the current device (256 on MI355X), queried once per device (callers
+// may be on several threads).
 inline int device_cu_count() {
-  static int n = 0;
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int n = cache[dev].load(std::memory_order_relaxed);
   if (!n) {
-    int dev = 0, v = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-      n = v;
-    else
-      n = 256;
+    int v = 0;
+    n = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0 ? v : 256;
+    cache[dev].store(n, std::memory_order_relaxed);
   }
   return n;
 }

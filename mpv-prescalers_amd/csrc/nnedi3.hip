@@ -601,10 +601,15 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
 #define MPX_NN(FM, TERMS, NSL, TPW, SC)                                                               \
   {                                                                                                   \
     auto kern = nnedi3_mfma<WH, AXIS, FM, TERMS, NSL, TPW, SC>;                                       \
-    static int per_cu = 0; /* per instance, queried once (no runtime queries while capturing) */      \
-    if (!per_cu &&                                                                                    \
-        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1)) \
-      per_cu = 1;                                                                                     \
+    /* per instance, queried once (no runtime queries while capturing); callers may be on */        \
+    /* several threads: an atomic cache (a race only repeats the same query)                */        \
+    static std::atomic<int> occ{0};                                                                   \
+    int per_cu = occ.load(std::memory_order_relaxed);                                                 \
+    if (!per_cu) {                                                                                    \
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) \
+        per_cu = 1;                                                                                   \
+      occ.store(per_cu, std::memory_order_relaxed);                                                   \
+    }                                                                                                 \
     const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
     const int grid = min(total, device_cu_count() * k);                                                      \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
