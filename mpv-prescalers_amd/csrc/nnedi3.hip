@@ -37,6 +37,7 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 
 constexpr int NX = 64;  // predicted pixels per block tile: NX x NY
 constexpr int NY = 4;
+constexpr int NYS = 8;  // rows of the persistent MFMA kernel's staged tile: two NY sub-tiles
 
 struct NPass {
   const void* src;
@@ -57,19 +58,20 @@ struct NPass {
   int tiles[kMaxTab + 1];
 };
 
-// The NX x NY tile t of a pass: frame, origin and that frame's geometry.
+// The NX x TY tile t of a pass: frame, origin and that frame's geometry.
 struct PassTile {
   int f, x0, y0, W, H;
   int64_t sp, dp;
 };
+template <int TY>
 __device__ __forceinline__ PassTile pass_tile(const NPass& p, int t) {
   PassTile r;
   if (!p.sized) {
-    const int nx = (p.W + NX - 1) / NX, per = nx * ((p.H + NY - 1) / NY);
+    const int nx = (p.W + NX - 1) / NX, per = nx * ((p.H + TY - 1) / TY);
     r.f = t / per;
     const int q = t - r.f * per;
     r.x0 = (q % nx) * NX;
-    r.y0 = (q / nx) * NY;
+    r.y0 = (q / nx) * TY;
     r.W = p.W;
     r.H = p.H;
     r.sp = p.src_pitch;
@@ -81,7 +83,7 @@ __device__ __forceinline__ PassTile pass_tile(const NPass& p, int t) {
   const int q = t - p.tiles[f], nx = (p.fW[f] + NX - 1) / NX;
   r.f = f;
   r.x0 = (q % nx) * NX;
-  r.y0 = (q / nx) * NY;
+  r.y0 = (q / nx) * TY;
   r.W = p.fW[f];
   r.H = p.fH[f];
   r.sp = p.fsp[f];
@@ -94,12 +96,12 @@ constexpr float kVarEps = 1.192092896e-7f;  // nnedi3-nns16-win8x4.hook:28
 // Window / tile geometry.  AXIS 0 = double_y (new pixel between rows y and y+1), window
 // dx in [-3,4] (8 long, outer index a), dy in [LO, LO+WH-1] (inner index b);
 // AXIS 1 = double_x, the transpose.  Canonical index k = a*WH + b for both.
-template <int WH, int AXIS>
+template <int WH, int AXIS, int TY = NY>
 struct Geo {
   static constexpr int K = 8 * WH;
   static constexpr int LO = -(WH / 2 - 1);
   static constexpr int TW = AXIS == 0 ? NX + 7 : NX + WH - 1;
-  static constexpr int TH = AXIS == 0 ? NY + WH - 1 : NY + 7;
+  static constexpr int TH = AXIS == 0 ? TY + WH - 1 : TY + 7;
   static constexpr int OX = AXIS == 0 ? -3 : LO;  // tile origin relative to block origin
   static constexpr int OY = AXIS == 0 ? LO : -3;
   __device__ static int at(int tx, int ty, int a, int b) {
@@ -108,8 +110,8 @@ struct Geo {
   __device__ static int centre(int tx, int ty) { return (ty - OY) * TW + tx - OX; }
 };
 
-template <int WH, int AXIS>
-using NTile = HaloTile<1, Geo<WH, AXIS>::TH, Geo<WH, AXIS>::TW, 256>;
+template <int WH, int AXIS, int TY = NY>
+using NTile = HaloTile<1, Geo<WH, AXIS, TY>::TH, Geo<WH, AXIS, TY>::TW, 256>;
 
 // mean, 1/std (0 below the variance threshold) and std of a K-sample window held in
 // registers; the window is replaced by its centred form, and by its centred, normalised
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
   using G = Geo<WH, AXIS>;
   constexpr int K = G::K;
   __shared__ float tile[G::TH * G::TW];
-  const PassTile pt = pass_tile(p, xcd_tile(blockIdx.x, total));
+  const PassTile pt = pass_tile<NY>(p, xcd_tile(blockIdx.x, total));
   const int f = pt.f;
   const int x0 = pt.x0, y0 = pt.y0;
   {
@@ -392,12 +394,14 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Block: 4 waves, tile NX x NY = 64 x 4 pixels, wave w owning pixel row w (pixel groups
-// 4w..4w+3 of 16).  The NT = nns/16 neuron tiles are split into NSL slices; TPW = NT/NSL
+// Block: 4 waves; a staged tile of NX x NYS = 64 x 8 pixels is computed as two sub-tiles
+// of NX x NY = 64 x 4, wave w owning pixel row w of each (pixel groups 4w..4w+3 of 16):
+// one halo load and one block barrier serve 512 pixels (win8x6 forms: 4-6 % faster than
+// 64 x 4 tiles).  The NT = nns/16 neuron tiles are split into NSL slices; TPW = NT/NSL
 // neuron tiles per wave, held in VGPRs for the whole launch.
 //  * NSL = 1 (nns <= 64): every wave runs all neurons for its own 64 pixels, so the
 //    normalised rows it writes to LDS are read back by the same wave and the softmax sums
-//    stay in registers: one block barrier per tile (after the halo tile lands).
+//    stay in registers: one block barrier per staged tile (after the halo tile lands).
 //  * NSL > 1: waves with the same neuron slice share the 16 pixel groups (4/NSL pixel
 //    slices); partial sums meet in LDS.
 // TERMS = 1: one product of 16-bit operands; TERMS = 2: high*high + high*low (split
@@ -412,7 +416,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
                                                       const void* __restrict__ w_lo, const int* __restrict__ wsc,
                                                       const float* __restrict__ bias, float xscale, float inv_acc,
                                                       int total_tiles) {
-  using G = Geo<WH, AXIS>;
+  using G = Geo<WH, AXIS, NYS>;
   constexpr int K = G::K;
   using M = Mma<FORM, K, TERMS>;
   using T = typename M::T;
@@ -501,9 +505,9 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     vs += __shfl_xor(vs, 32);
   };
 
-  NTile<WH, AXIS> ht;
+  NTile<WH, AXIS, NYS> ht;
   auto issue = [&](int id) {
-    const PassTile q = pass_tile(p, id);
+    const PassTile q = pass_tile<NYS>(p, id);
     ht.issue(frame_src(p, q.f), p.src_dt, q.sp, 0, q.W, q.H, q.x0 + G::OX, q.y0 + G::OY);
   };
   // launch-order index L = blockIdx + i*gridDim walks [0, total); tile = xcd_tile(L)
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   int buf = 0;
   for (int L = blockIdx.x; L < total_tiles; L += gridDim.x, buf ^= 1) {
     const int tileid = xcd_tile(L, total_tiles);
-    const PassTile pt = pass_tile(p, tileid);
+    const PassTile pt = pass_tile<NYS>(p, tileid);
     const int f = pt.f;
     const int y0 = pt.y0, x0 = pt.x0;
     // The other buffer was last read two tiles ago, before the barrier below was last
@@ -520,61 +524,68 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     __syncthreads();
     if (L + static_cast<int>(gridDim.x) < total_tiles) issue(xcd_tile(L + gridDim.x, total_tiles));
 
-    // one thread per pixel: normalised window -> 16-bit rows (hi, lo), zero padded to KP
-    const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;
-    float m, sd;
-    float orig;
-    {
-      const float* tl = tile[buf];
-      float xv[K];
+    // the staged tile holds NYS = 2 * NY rows: two sub-tiles of NX x NY pixels, one
+    // pixel per thread each, sharing one halo load and one block barrier
+#pragma unroll 1
+    for (int sub = 0; sub < NYS / NY; ++sub) {
+      // one thread per pixel: normalised window -> 16-bit rows (hi, lo), zero padded to KP
+      const int tx = threadIdx.x % NX, ty = threadIdx.x / NX + sub * NY;
+      float m, sd;
+      float orig;
+      {
+        const float* tl = tile[buf];
+        float xv[K];
 #pragma unroll
-      for (int a = 0; a < 8; ++a)
+        for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int b = 0; b < WH; ++b) xv[a * WH + b] = tl[G::at(tx, ty, a, b)];
-      orig = tl[G::centre(tx, ty)];
-      float is;
-      normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
-      M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED, is);
-    }
+          for (int b = 0; b < WH; ++b) xv[a * WH + b] = tl[G::at(tx, ty, a, b)];
+        orig = tl[G::centre(tx, ty)];
+        float is;
+        normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
+        M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED, is);
+      }
 
-    float wsum = 0.f, vsum = 0.f;
-    if constexpr (NSL == 1) {
-      // this wave's own rows: groups 4*wave + j; after group j, the lanes of quad j hold
-      // their own pixel's sums
-      wave_lds_sync();
+      float wsum = 0.f, vsum = 0.f;
+      if constexpr (NSL == 1) {
+        // this wave's own rows: groups 4*wave + j; after group j, the lanes of quad j hold
+        // their own pixel's sums
+        wave_lds_sync();
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        float ws, vs;
-        group_sums(wave * 4 + j, ws, vs);
-        if (quad == j) {
-          wsum = ws;
-          vsum = vs;
+        for (int j = 0; j < 4; ++j) {
+          float ws, vs;
+          group_sums(wave * 4 + j, ws, vs);
+          if (quad == j) {
+            wsum = ws;
+            vsum = vs;
+          }
+        }
+      } else {
+        // (also orders the previous sub-tile's reads of `part` before this one's writes)
+        __syncthreads();
+        for (int g = pslice; g < NGROUP; g += PSL) {
+          float ws, vs;
+          group_sums(g, ws, vs);
+          if (quad == 0) {
+            part[nslice][g * 16 + row][0] = ws;
+            part[nslice][g * 16 + row][1] = vs;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NSL; ++s) {
+          wsum += part[s][threadIdx.x][0];
+          vsum += part[s][threadIdx.x][1];
         }
       }
-    } else {
-      __syncthreads();
-      for (int g = pslice; g < NGROUP; g += PSL) {
-        float ws, vs;
-        group_sums(g, ws, vs);
-        if (quad == 0) {
-          part[nslice][g * 16 + row][0] = ws;
-          part[nslice][g * 16 + row][1] = vs;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < NSL; ++s) {
-        wsum += part[s][threadIdx.x][0];
-        vsum += part[s][threadIdx.x][1];
-      }
-    }
 
-    const int x = x0 + tx, y = y0 + ty;
-    if (x < pt.W && y < pt.H) {
-      const float pred = fminf(fmaxf(m + 5.0f * vsum * __builtin_amdgcn_rcpf(wsum) * sd, 0.f), 1.f);
-      store_result<AXIS>(p, frame_dst(p, f), pt.dp, x, y, orig, pred);
+      const int x = x0 + tx, y = y0 + ty;
+      if (x < pt.W && y < pt.H) {
+        const float pred = fminf(fmaxf(m + 5.0f * vsum * __builtin_amdgcn_rcpf(wsum) * sd, 0.f), 1.f);
+        store_result<AXIS>(p, frame_dst(p, f), pt.dp, x, y, orig, pred);
+      }
+      // this sub-tile's row reads before the next sub-tile's (or tile's) writes
+      if constexpr (NSL == 1) wave_lds_sync();
     }
-    if constexpr (NSL == 1) wave_lds_sync();  // this tile's row reads before the next tile's writes
   }
 }
 
@@ -582,12 +593,15 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
 
 template <int WH, int AXIS>
 static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
-  int total = ((p.W + NX - 1) / NX) * ((p.H + NY - 1) / NY) * p.batch;
-  if (p.sized) {
+  // tiles of NX x ty pixels (ty = NY for the plain kernel, NYS for the persistent MFMA
+  // kernel); fills the per-frame prefix of sized launches
+  auto count_tiles = [&](int ty) {
+    if (!p.sized) return ((p.W + NX - 1) / NX) * ((p.H + ty - 1) / ty) * p.batch;
     p.tiles[0] = 0;
-    for (int f = 0; f < p.ntab_src; ++f) p.tiles[f + 1] = p.tiles[f] + ((p.fW[f] + NX - 1) / NX) * ((p.fH[f] + NY - 1) / NY);
-    total = p.tiles[p.ntab_src];
-  }
+    for (int f = 0; f < p.ntab_src; ++f) p.tiles[f + 1] = p.tiles[f] + ((p.fW[f] + NX - 1) / NX) * ((p.fH[f] + ty - 1) / ty);
+    return p.tiles[p.ntab_src];
+  };
+  int total = count_tiles(nw.precision == 0 ? NY : NYS);
   if (nw.precision == 0) {
     hipLaunchKernelGGL((nnedi3_valu<WH, AXIS>), dim3(total), dim3(256), 0, st, p, static_cast<const float*>(nw.w),
                        nw.nns, total);
