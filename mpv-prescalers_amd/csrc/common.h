@@ -246,6 +246,21 @@ __device__ __host__ constexpr double gauss_w(int i) {
        : N == 6 ? kGauss6[i] : N == 7 ? kGauss7[i] : kGauss8[i];
 }
 
+// x / b for a constant b, rounded exactly as the division rounds (to nearest), given
+// rb = RN(1/b).  FP64 takes three operations instead of the division's eleven (one of
+// them a quarter-rate reciprocal): q0 = x * rb lies within one ulp of x/b, the residual
+// x - b*q0 is exact under fma, and one corrected step gives RN(x/b) (Markstein's theorem;
+// the operands here are far from underflow and overflow).  A zero quotient may come out
+// +0 where the division gives -0, which no caller can tell apart (squares and sums).
+template <typename T>
+__device__ __forceinline__ T div_const(T x, T b, T rb) {
+  if constexpr (sizeof(T) == 8) {
+    const T q0 = x * rb;
+    return __builtin_fma(__builtin_fma(-q0, b, x), rb, q0);
+  }
+  return x / b;
+}
+
 // Derivative along one window line at position p, in the shaders' formula forms
 // (ravu-lite-r3.hook:50 central; ravu-r3.hook:63 five-point; ravu-zoom-r2.hook:50
 // forward/backward at the window border).  `s(k)` returns the line's k-th sample.
@@ -256,12 +271,13 @@ __device__ __forceinline__ T deriv(Line s, int p) {
   if (p == N - 1) return s(N - 1) - s(N - 2);
   // FP32 (the fast path, whose decisions are trusted only away from bucket boundaries)
   // multiplies by the reciprocals: exact for 1/2, within one rounding of the quotient for
-  // 1/12.  FP64 (the re-decision) divides, as the oracle does.
+  // 1/12.  FP64 (the re-decision) gives the correctly rounded quotient, as the oracle's
+  // division does.
   if (FIVE && p >= 2 && p + 2 <= N - 1) {
     if constexpr (sizeof(T) == 4)  // same value, fused: (s[p-2] - s[p+2]) + 8 (s[p+1] - s[p-1])
       return fmaf(8.0f, s(p + 1) - s(p - 1), s(p - 2) - s(p + 2)) * (1.0f / 12.0f);
     const T num = -s(p + 2) + T(8) * s(p + 1) - T(8) * s(p - 1) + s(p - 2);
-    return num / T(12);
+    return div_const(num, T(12), T(1) / T(12));
   }
   if constexpr (sizeof(T) == 4) return (s(p + 1) - s(p - 1)) * T(0.5);
   return (s(p + 1) - s(p - 1)) / T(2);
@@ -341,13 +357,13 @@ __device__ __forceinline__ Bucket hash_reference(T a, T b, T d) {
   T sqrtL1 = sqrt(L1);
   T sqrtL2 = sqrt(L2);  // NaN when L2 < 0, as in the reference
   T at = atan2(L1 - a, b) + pi;
-  T theta = at - pi * floor(at / pi);  // GLSL mod
+  T theta = at - pi * floor(div_const(at, pi, T(1) / pi));  // GLSL mod
   if (fabs(b) < eps) theta = T(0);
   T ssum = sqrtL1 + sqrtL2;
   T mu = (sqrtL1 - sqrtL2) / ssum;
   if (ssum < eps) mu = T(0);
   Bucket bk;
-  bk.angle = static_cast<int>(floor(theta * T(24) / pi));
+  bk.angle = static_cast<int>(floor(div_const(theta * T(24), pi, T(1) / pi)));
   bk.strength = strength_of<RULE, T>(sqrtL1);
   bk.coherence = mu >= T(0.5) ? 2 : (mu >= T(0.25) ? 1 : 0);  // NaN -> 0
   return bk;
