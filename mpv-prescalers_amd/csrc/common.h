@@ -126,6 +126,16 @@ inline int launch_tiles(Io& io) {
   return io.tiles[io.ntab];
 }
 
+// v / m for an integer sample v (0..65535) and full scale m = 2^bits - 1, rounded exactly
+// as the division rounds, given rm = RN(1/m): q0 = v * rm lies within one ulp of v/m, the
+// residual v - m*q0 is exact under fma, and one corrected step gives RN(v/m) (Markstein's
+// theorem).  Three operations instead of the division's ten; equal to the division for
+// every 16-bit v and every bits 1..16 (checked exhaustively).
+__device__ __forceinline__ float unorm_quotient(float v, float m, float rm) {
+  const float q0 = v * rm;
+  return fmaf(fmaf(-q0, m, v), rm, q0);
+}
+
 __device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, float in_max) {
   switch (dt) {
     case DT_F32: return static_cast<const float*>(p)[idx];
@@ -188,6 +198,8 @@ struct HaloTile {
     }
   }
   __device__ __forceinline__ void commit(float* tile, int dt, float in_max) const {
+    // integer samples: value / full scale, one division per thread for the reciprocal
+    const float rmax = dt == DT_U8 || dt == DT_U16 ? 1.0f / in_max : 1.0f;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = static_cast<int>(threadIdx.x) + e * NT;
@@ -197,8 +209,8 @@ struct HaloTile {
         switch (dt) {
           case DT_F32: f = __uint_as_float(v[e]); break;
           case DT_F16: f = __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu))); break;
-          case DT_U8: f = static_cast<float>(w & 0xffu) / in_max; break;  // value / full scale
-          default: f = static_cast<float>(w & 0xffffu) / in_max; break;
+          case DT_U8: f = unorm_quotient(static_cast<float>(w & 0xffu), in_max, rmax); break;
+          default: f = unorm_quotient(static_cast<float>(w & 0xffffu), in_max, rmax); break;
         }
         tile[i] = f;
       }

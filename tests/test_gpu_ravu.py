@@ -155,3 +155,21 @@ def test_ravu_zoom_ar_r3_with_supplied_table(tmp_path):
     ref, _ = ravu_zoom(img.astype(np.float64), 3, 120, 72, ar=True,
                        ar_lut=table.astype(np.float16).astype(np.float64))
     assert np.abs(out - ref).max() <= 5e-5
+
+
+@pytest.mark.parametrize("name,target", [("ravu-r4-yuv", None), ("ravu-r3-rgb", None), ("ravu-3x-r3-yuv", None),
+                                         ("ravu-zoom-ar-r2-rgb", (250, 141)), ("ravu-lite-ar-r4", None)])
+@pytest.mark.parametrize("bits", [10, 16])
+def test_ravu_integer_input_equals_float(name, target, bits):
+    """Integer planes in 16-bit containers (value / (2^bits - 1), as mpv's sampler
+    normalises them): each output equals the run on the FP32 quotients, bit for bit."""
+    import mpx
+    planes = 1 if name.endswith(("-r4", "-r3")) else 3
+    full = 2 ** bits - 1
+    u16 = np.stack([np.round(_frame(100, 57, seed=60 + s, planes=planes, bits=bits) * full) for s in range(2)])
+    x = torch.from_numpy(u16.astype(np.int32)).to(torch.uint16).cuda()
+    out = mpx.upscale(name, x, target=target, out_dtype=torch.float32, bits=bits)
+    for i in range(2):
+        xf = torch.from_numpy(u16[i].astype(np.float32) / np.float32(full)).cuda()
+        single = mpx.upscale(name, xf, target=target)
+        assert torch.equal(out[i], single), (name, bits, i)

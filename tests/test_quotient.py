@@ -62,3 +62,37 @@ def test_oracle_rounding_is_what_a_plain_multiply_misses():
     xs = _random_doubles(rng, 4000, -10, 10)
     assert any(x * (1.0 / 12.0) != x / 12.0 for x in xs)
     assert all(div_const(x, 12.0) == x / 12.0 for x in xs)
+
+
+def _fma32(a, b, c):
+    """Exact float32 fma, vectorised: the float64 product of two float32 values is exact;
+    where the float64 sum is exact too (checked by two-sum) one rounding to float32 is the
+    fma's; the other elements are rounded once from exact rationals."""
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    c = c.astype(np.float64)
+    p = a * b
+    s = p + c
+    bp = s - c
+    err = (p - bp) + (c - (s - bp))  # two-sum error of p + c
+    out = s.astype(np.float32)
+    for i in np.flatnonzero(err != 0.0):
+        out[i] = np.float32(float(Fraction(float(a[i])) * Fraction(float(b[i])) + Fraction(float(c[i]))))
+    return out
+
+
+def test_unorm_quotient_equals_division_exhaustively():
+    # integer samples (U8 / U16 inputs) are value / (2^bits - 1) in every staged tile
+    # (common.h, unorm_quotient): every 16-bit value, every full scale
+    v = np.arange(65536, dtype=np.float32)
+    for bits in range(1, 17):
+        m = np.float32(2 ** bits - 1)
+        rm = np.float32(1.0) / m
+        q0 = v * rm
+        r = _fma32(-q0, np.full_like(v, m), v)
+        q = _fma32(r, np.full_like(v, rm), q0)
+        want = v / m
+        bad = np.flatnonzero(q.view(np.uint32) != want.view(np.uint32))
+        assert bad.size == 0, (bits, v[bad[:5]], q[bad[:5]], want[bad[:5]])
+        if bits >= 8:  # the correction step matters: the plain product misses the division
+            assert np.any(q0.view(np.uint32) != want.view(np.uint32))
