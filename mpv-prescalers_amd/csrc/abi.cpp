@@ -322,6 +322,7 @@ int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
   d.in_dt = plan->in_type;
   d.out_dt = plan->out_type;
   d.in_max = plan->in_max;
+  d.in_rmax = 1.0f / d.in_max;
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = io->batch;
@@ -357,6 +358,7 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
   d.in_dt = plan->in_type;
   d.out_dt = plan->out_type;
   d.in_max = plan->in_max;
+  d.in_rmax = 1.0f / d.in_max;
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = n;
@@ -401,6 +403,7 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
   d.in_dt = plan->in_type;
   d.out_dt = plan->out_type;
   d.in_max = plan->in_max;
+  d.in_rmax = 1.0f / d.in_max;
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = n;
@@ -492,6 +495,7 @@ int mpx_resample(const mpx_io* io, int in_w, int in_h, int in_type, int bits, in
   d.in_dt = in_type;
   d.out_dt = out_type;
   d.in_max = in_type == MPX_U8 ? 255.f : (in_type == MPX_U16 ? static_cast<float>((1 << bits) - 1) : 1.f);
+  d.in_rmax = 1.0f / d.in_max;
   d.W = in_w;
   d.H = in_h;
   d.batch = io->batch;

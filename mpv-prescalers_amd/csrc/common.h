@@ -29,6 +29,7 @@ struct Io {
   int64_t bucket_bstride;
   int in_dt, out_dt;
   float in_max;       // integer full-scale value for U8/U16 inputs (255, 1023, ...)
+  float in_rmax;      // RN(1 / in_max), computed on the host
   int W, H;           // input plane size
   int batch;
   // Frames of different sizes in one launch (sized != 0; ntab frames through the pointer
@@ -197,9 +198,8 @@ struct HaloTile {
       sh[e] = static_cast<uint32_t>(a & 3) * 8;
     }
   }
-  __device__ __forceinline__ void commit(float* tile, int dt, float in_max) const {
-    // integer samples: value / full scale, one division per thread for the reciprocal
-    const float rmax = dt == DT_U8 || dt == DT_U16 ? 1.0f / in_max : 1.0f;
+  // in_rmax = RN(1 / in_max) (from the host): integer samples cost three operations each
+  __device__ __forceinline__ void commit(float* tile, int dt, float in_max, float in_rmax) const {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = static_cast<int>(threadIdx.x) + e * NT;
@@ -209,8 +209,8 @@ struct HaloTile {
         switch (dt) {
           case DT_F32: f = __uint_as_float(v[e]); break;
           case DT_F16: f = __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu))); break;
-          case DT_U8: f = unorm_quotient(static_cast<float>(w & 0xffu), in_max, rmax); break;
-          default: f = unorm_quotient(static_cast<float>(w & 0xffffu), in_max, rmax); break;
+          case DT_U8: f = unorm_quotient(static_cast<float>(w & 0xffu), in_max, in_rmax); break;
+          default: f = unorm_quotient(static_cast<float>(w & 0xffffu), in_max, in_rmax); break;
         }
         tile[i] = f;
       }
@@ -274,7 +274,7 @@ __device__ __forceinline__ T div_const(T x, T b, T rb) {
 }
 
 // Derivative along one window line at position p, in the shaders' formula forms
-// (ravu-lite-r3.hook:50 central; ravu-r3.hook:63 five-point; ravu-zoom-r2.hook:50
+// (ravu-lite-r3.hook:50 central; ravu-r3.hook:64 five-point; ravu-zoom-r2.hook:50
 // forward/backward at the window border).  `s(k)` returns the line's k-th sample.
 template <int N, bool FIVE, typename T, typename Line>
 __device__ __forceinline__ T deriv(Line s, int p) {
@@ -295,11 +295,29 @@ __device__ __forceinline__ T deriv(Line s, int p) {
   return (s(p + 1) - s(p - 1)) / T(2);
 }
 
+// The same derivatives without their stencil's scale (FP32 fast path), and that scale:
+// 1 at the window border (forward/backward), 1/12 for the five-point form, 1/2 central.
+template <int N, bool FIVE>
+__device__ __host__ constexpr double deriv_scale(int p) {
+  return (p == 0 || p == N - 1) ? 1.0 : (FIVE && p >= 2 && p + 2 <= N - 1) ? 1.0 / 12.0 : 0.5;
+}
+template <int N, bool FIVE, typename Line>
+__device__ __forceinline__ float deriv_unscaled(Line s, int p) {
+  if (p == 0) return s(1) - s(0);
+  if (p == N - 1) return s(N - 1) - s(N - 2);
+  if (FIVE && p >= 2 && p + 2 <= N - 1) return fmaf(8.0f, s(p + 1) - s(p - 1), s(p - 2) - s(p + 2));
+  return s(p + 1) - s(p - 1);
+}
+
 // abd = sum over the gradient grid of (gx*gx, gx*gy, gy*gy) * G, grid X outer, Y inner
 // (ravu-lite-r3.hook:48-76).  get(i) returns window sample i = X*N + Y.  FP32 (the fast
-// path, trusted only away from bucket boundaries) fuses multiply-adds; FP64 (the
-// re-decision) keeps every rounding step of the formula, like the oracle.
-template <int N, bool FIVE, typename T, typename Get>
+// path, trusted only away from bucket boundaries) fuses multiply-adds; with FOLD it takes
+// the derivatives unscaled and folds each point's stencil scales into its Gaussian weight
+// (compile-time constants), and where gx and gy carry the same scale one product w'*gx
+// serves both a and b (fewer operations; used where it does not raise the kernel's
+// register count).  FP64 (the re-decision) keeps every rounding step of the formula,
+// like the oracle.
+template <int N, bool FIVE, typename T, bool FOLD, typename Get>
 __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
 #pragma clang fp contract(off)
   constexpr int P0 = N <= 4 ? 0 : 1;
@@ -310,6 +328,16 @@ __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
 #pragma unroll
     for (int Y = P0; Y < P1; ++Y) {
       const int gi = (X - P0) * (P1 - P0) + (Y - P0);
+      if constexpr (sizeof(T) == 4 && FOLD) {
+        const float gx = deriv_unscaled<N, FIVE>([&](int k) { return get(k * N + Y); }, X);
+        const float gy = deriv_unscaled<N, FIVE>([&](int k) { return get(X * N + k); }, Y);
+        const double cx = deriv_scale<N, FIVE>(X), cy = deriv_scale<N, FIVE>(Y), w = gauss_w<N>(gi);
+        const float u = gx * static_cast<float>(w * cx * cx);
+        a = fmaf(u, gx, a);
+        b = cx == cy ? fmaf(u, gy, b) : fmaf(gx * static_cast<float>(w * cx * cy), gy, b);
+        d = fmaf(gy * static_cast<float>(w * cy * cy), gy, d);
+        continue;
+      }
       T gx = deriv<N, FIVE, T>([&](int k) { return get(k * N + Y); }, X);
       T gy = deriv<N, FIVE, T>([&](int k) { return get(X * N + k); }, Y);
       const T w = T(gauss_w<N>(gi));
@@ -445,16 +473,16 @@ __device__ __forceinline__ int bucket_row(Bucket bk, int rows) {
 
 // Hash of one site: FP32, re-decided in FP64 when near a boundary.  getf(i) -> float
 // and getd(i) -> double return window sample i (getd widens the same stored values, and
-// for RGB forms it recomputes the BT.709 luma in FP64).
-template <int N, bool FIVE, int RULE, typename GetF, typename GetD>
+// for RGB forms it recomputes the BT.709 luma in FP64).  FOLD picks the FP32 tensor form.
+template <int N, bool FIVE, int RULE, bool FOLD = false, typename GetF, typename GetD>
 __device__ __forceinline__ Bucket ravu_hash(GetF getf, GetD getd) {
   float a, b, d;
-  structure_tensor<N, FIVE, float>(getf, a, b, d);
+  structure_tensor<N, FIVE, float, FOLD>(getf, a, b, d);
   bool near = false;
   Bucket bk = hash_fast<RULE>(a, b, d, &near);
   if (near) {
     double ad, bd, dd;
-    structure_tensor<N, FIVE, double>(getd, ad, bd, dd);
+    structure_tensor<N, FIVE, double, false>(getd, ad, bd, dd);
     bk = hash_reference<RULE, double>(ad, bd, dd);
   }
   return bk;

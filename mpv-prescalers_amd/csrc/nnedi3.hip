@@ -47,7 +47,7 @@ struct NPass {
   int ntab_src, ntab_dst;
   int64_t src_pitch, src_bstride, dst_pitch, dst_bstride;
   int src_dt, dst_dt;
-  float src_max;
+  float src_max, src_rmax;  // integer full scale and RN(1 / full scale)
   int W, H;  // source plane
   int batch;
   // frames of different sizes (sized != 0, ntab_src = ntab_dst frames): each frame's
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void nnedi3_valu(NPass p, const float* __restr
   {
     NTile<WH, AXIS> ht;
     ht.issue(frame_src(p, f), p.src_dt, pt.sp, 0, pt.W, pt.H, x0 + G::OX, y0 + G::OY);
-    ht.commit(tile, p.src_dt, p.src_max);
+    ht.commit(tile, p.src_dt, p.src_max, p.src_rmax);
   }
   __syncthreads();
   const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     const int y0 = pt.y0, x0 = pt.x0;
     // The other buffer was last read two tiles ago, before the barrier below was last
     // passed by every wave.
-    ht.commit(tile[buf], p.src_dt, p.src_max);
+    ht.commit(tile[buf], p.src_dt, p.src_max, p.src_rmax);
     __syncthreads();
     if (L + static_cast<int>(gridDim.x) < total_tiles) issue(xcd_tile(L + gridDim.x, total_tiles));
 
@@ -678,6 +678,7 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
     p.src_bstride = io.in_bstride;
     p.src_dt = io.in_dt;
     p.src_max = io.in_max;
+    p.src_rmax = io.in_rmax;
     p.W = io.W;
     p.H = io.H;
     p.batch = io.batch;
@@ -722,6 +723,7 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
       p.src_bstride = tmp_frame;
       p.src_dt = DT_F32;
       p.src_max = 1.f;
+      p.src_rmax = 1.f;
       p.H = 2 * io.H;
     } else {
       p.src = io.in;
@@ -729,6 +731,7 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
       p.src_bstride = io.in_bstride;
       p.src_dt = io.in_dt;
       p.src_max = io.in_max;
+      p.src_rmax = io.in_rmax;
       p.H = io.H;
       if (io.ntab) {
         p.ntab_src = io.ntab;

@@ -25,11 +25,11 @@ constexpr int RT_Y = 4;
 // Stage P planes of a frame into LDS: tile[c][r][col] = plane c at
 // (clamp(ox + col), clamp(oy + r)), PH x PW per plane; all loads in flight at once.
 template <int P, int PH, int PW>
-__device__ __forceinline__ void stage_planes(float* tile, const void* base, int dt, float in_max, int64_t pitch,
-                                             int64_t cstride, int W, int H, int ox, int oy) {
+__device__ __forceinline__ void stage_planes(float* tile, const void* base, int dt, float in_max, float in_rmax,
+                                             int64_t pitch, int64_t cstride, int W, int H, int ox, int oy) {
   HaloTile<P, PH, PW, 256> h;
   h.issue(base, dt, pitch, cstride, W, H, ox, oy);
-  h.commit(tile, dt, in_max);
+  h.commit(tile, dt, in_max, in_rmax);
 }
 
 // Planes staged per tile: the P input planes, plus for rgb a fourth holding the BT.709
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - HALO, y0 - HALO);
+  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - HALO, y0 - HALO);
   __syncthreads();
   if constexpr (CH == 2) {
     stage_luma<CH>(tile, PN);
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
+  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
                   y0 - (R - 1));
   __syncthreads();
   if constexpr (CH == 2) {
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
   auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };
-  const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(tile, off(i), PN); },
+  const Bucket bk = ravu_hash<N, true, RULE_LOG2, true>([&](int i) { return luma_f<CH>(tile, off(i), PN); },
                                                   [&](int i) { return luma_d<CH>(tile, off(i), PN); });
   const int row = bucket_row<RULE_LOG2>(bk, 648);
   if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * ft.W + x] = row;
@@ -283,8 +283,8 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
     HaloTile<P, PH, PW, 256> a, b;  // both tiles' loads in flight together
     a.issue(in, io.in_dt, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - R, y0 - R);
     b.issue(int11 + f * int11_frame, DT_F32, ft.W, plane, ft.W, ft.H, x0 - R, y0 - R);
-    a.commit(hk, io.in_dt, io.in_max);
-    b.commit(i11, DT_F32, 1.f);
+    a.commit(hk, io.in_dt, io.in_max, io.in_rmax);
+    b.commit(i11, DT_F32, 1.f, 1.f);
   }
   __syncthreads();
   if constexpr (CH == 2) {
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
     const int px = lat2x<N, STEP>(X, Y), py = lat2y<N, STEP>(X, Y);
     return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
   };
-  const Bucket bk = ravu_hash<N, true, RULE_LOG2>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
+  const Bucket bk = ravu_hash<N, true, RULE_LOG2, true>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
                                                   [&](int i) { return luma_d<CH>(src(i), 0, PN); });
   const int row = bucket_row<RULE_LOG2>(bk, 648);
   if (io.buckets) io.buckets[f * io.bucket_bstride + (STEP - 1) * plane + static_cast<int64_t>(y) * ft.W + x] = row;

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   {
     HaloTile<1, PH, PW, 256> h;
     h.issue(in, io.in_dt, ft.in_pitch, 0, ft.W, ft.H, x0 - HALO, y0 - HALO);
-    h.commit(tile, io.in_dt, io.in_max);
+    h.commit(tile, io.in_dt, io.in_max, io.in_rmax);
   }
   __syncthreads();
 

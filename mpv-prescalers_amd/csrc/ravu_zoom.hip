@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   {
     HaloTile<P, PH, PW, 256> h;
     h.issue(in, io.in_dt, io.in_pitch, io.in_cstride, io.W, io.H, bx0 - (R - 1), by0 - (R - 1));
-    h.commit(tile, io.in_dt, io.in_max);
+    h.commit(tile, io.in_dt, io.in_max, io.in_rmax);
   }
   __syncthreads();
   // hash every base pixel the tile's outputs use (unclamped LUT row: bucket 9-row blocks
