@@ -311,13 +311,16 @@ __device__ __forceinline__ float deriv_unscaled(Line s, int p) {
 
 // abd = sum over the gradient grid of (gx*gx, gx*gy, gy*gy) * G, grid X outer, Y inner
 // (ravu-lite-r3.hook:48-76).  get(i) returns window sample i = X*N + Y.  FP32 (the fast
-// path, trusted only away from bucket boundaries) fuses multiply-adds; with FOLD it takes
-// the derivatives unscaled and folds each point's stencil scales into its Gaussian weight
+// path, trusted only away from bucket boundaries) fuses multiply-adds; FP64 (the
+// re-decision) keeps every rounding step of the formula, like the oracle.
+// LEAN (the full-RAVU kernels, whose windows are largest) trims both: FP32 takes the
+// derivatives unscaled and folds each point's stencil scales into its Gaussian weight
 // (compile-time constants), and where gx and gy carry the same scale one product w'*gx
-// serves both a and b (fewer operations; used where it does not raise the kernel's
-// register count).  FP64 (the re-decision) keeps every rounding step of the formula,
-// like the oracle.
-template <int N, bool FIVE, typename T, bool FOLD, typename Get>
+// serves both a and b; FP64 is computed one derivative at a time (scheduling barriers,
+// the partial sums pinned), so the rare re-decision does not hold the window in FP64
+// registers and set the occupancy of every wave (r4: 122 -> 58 VGPRs).  The sums keep
+// the oracle's order either way.
+template <int N, bool FIVE, typename T, bool LEAN, typename Get>
 __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
 #pragma clang fp contract(off)
   constexpr int P0 = N <= 4 ? 0 : 1;
@@ -328,7 +331,7 @@ __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
 #pragma unroll
     for (int Y = P0; Y < P1; ++Y) {
       const int gi = (X - P0) * (P1 - P0) + (Y - P0);
-      if constexpr (sizeof(T) == 4 && FOLD) {
+      if constexpr (sizeof(T) == 4 && LEAN) {
         const float gx = deriv_unscaled<N, FIVE>([&](int k) { return get(k * N + Y); }, X);
         const float gy = deriv_unscaled<N, FIVE>([&](int k) { return get(X * N + k); }, Y);
         const double cx = deriv_scale<N, FIVE>(X), cy = deriv_scale<N, FIVE>(Y), w = gauss_w<N>(gi);
@@ -339,6 +342,10 @@ __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
         continue;
       }
       T gx = deriv<N, FIVE, T>([&](int k) { return get(k * N + Y); }, X);
+      if constexpr (sizeof(T) == 8 && LEAN) {
+        asm volatile("" : "+v"(gx));
+        __builtin_amdgcn_sched_barrier(0);
+      }
       T gy = deriv<N, FIVE, T>([&](int k) { return get(X * N + k); }, Y);
       const T w = T(gauss_w<N>(gi));
       if constexpr (sizeof(T) == 4) {  // fast path: fused accumulation
@@ -349,6 +356,10 @@ __device__ __forceinline__ void structure_tensor(Get get, T& a, T& b, T& d) {
         a = a + gx * gx * w;
         b = b + gx * gy * w;
         d = d + gy * gy * w;
+        if constexpr (LEAN) {
+          asm volatile("" : "+v"(a), "+v"(b), "+v"(d));
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
   }
@@ -473,16 +484,16 @@ __device__ __forceinline__ int bucket_row(Bucket bk, int rows) {
 
 // Hash of one site: FP32, re-decided in FP64 when near a boundary.  getf(i) -> float
 // and getd(i) -> double return window sample i (getd widens the same stored values, and
-// for RGB forms it recomputes the BT.709 luma in FP64).  FOLD picks the FP32 tensor form.
-template <int N, bool FIVE, int RULE, bool FOLD = false, typename GetF, typename GetD>
+// for RGB forms it recomputes the BT.709 luma in FP64).  LEAN picks the tensor forms.
+template <int N, bool FIVE, int RULE, bool LEAN = false, typename GetF, typename GetD>
 __device__ __forceinline__ Bucket ravu_hash(GetF getf, GetD getd) {
   float a, b, d;
-  structure_tensor<N, FIVE, float, FOLD>(getf, a, b, d);
+  structure_tensor<N, FIVE, float, LEAN>(getf, a, b, d);
   bool near = false;
   Bucket bk = hash_fast<RULE>(a, b, d, &near);
   if (near) {
     double ad, bd, dd;
-    structure_tensor<N, FIVE, double, false>(getd, ad, bd, dd);
+    structure_tensor<N, FIVE, double, LEAN>(getd, ad, bd, dd);
     bk = hash_reference<RULE, double>(ad, bd, dd);
   }
   return bk;
