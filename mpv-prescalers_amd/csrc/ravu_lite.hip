@@ -27,6 +27,9 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   constexpr int NTEX = (N * N + 1) / 2;
   constexpr int C = (N * N - 1) / 2;
   __shared__ float tile[PH * PW];
+  // AR: per staged sample (l + 0.1)^32, (1.1 - l)^32 and the ^33 terms, computed once per
+  // sample instead of once per pixel whose anti-ringing window reads it (up to 13)
+  __shared__ float4 pw[AR ? PH * PW : 1];
 
   const FrameTile ft = frame_tile<LITE_TX, LITE_TY>(io, blockIdx.x, total);
   const int frame = ft.f;
@@ -38,6 +41,18 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
     HaloTile<1, PH, PW, 256> h;
     h.issue(in, io.in_dt, ft.in_pitch, 0, ft.W, ft.H, x0 - HALO, y0 - HALO);
     h.commit(tile, io.in_dt, io.in_max, io.in_rmax);
+  }
+  if constexpr (AR) {
+    // each thread takes the samples it staged itself (the commit's index pattern), so the
+    // one block barrier below orders both tables
+    for (int i = threadIdx.x; i < PH * PW; i += 256) {
+      const float l = tile[i];
+      const float p0 = 0.1f + l, m0 = 1.1f - l;
+      float p32 = p0, m32 = m0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) { p32 *= p32; m32 *= m32; }
+      pw[i] = make_float4(p32, m32, p32 * p0, m32 * m0);
+    }
   }
   __syncthreads();
 
@@ -90,22 +105,16 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
       const int X = i / N - HALO, Y = i % N - HALO;
       if (X * X + Y * Y <= 4) {  // anti-ringing taps (ravu-lite-ar-r3.hook:95-181)
         const float4 wg = make_float4(fmaxf(w.x, 0.f), fmaxf(w.y, 0.f), fmaxf(w.z, 0.f), fmaxf(w.w, 0.f));
-        float p0 = 0.1f + li, m0 = 1.1f - li;
-        float p32 = p0, m32 = m0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) { p32 *= p32; m32 *= m32; }
-        const float p33 = p32 * p0, m33 = m32 * m0;
+        const float4 pa = pw[(ty + i % N) * PW + tx + i / N];  // sample i's powers
+        const float p32 = pa.x, m32 = pa.y, p33 = pa.z, m33 = pa.w;
         hi.x += p32 * wg.x; hi.y += p32 * wg.y; hi.z += p32 * wg.z; hi.w += p32 * wg.w;
         lo.x += m32 * wg.x; lo.y += m32 * wg.y; lo.z += m32 * wg.z; lo.w += m32 * wg.w;
         hi2.x += p33 * wg.x; hi2.y += p33 * wg.y; hi2.z += p33 * wg.z; hi2.w += p33 * wg.w;
         lo2.x += m33 * wg.x; lo2.y += m33 * wg.y; lo2.z += m33 * wg.z; lo2.w += m33 * wg.w;
         if (i < C) {
-          const float lj = s[N * N - 1 - i];
-          float p1 = 0.1f + lj, m1 = 1.1f - lj;
-          float q32 = p1, n32 = m1;
-#pragma unroll
-          for (int k = 0; k < 5; ++k) { q32 *= q32; n32 *= n32; }
-          const float q33 = q32 * p1, n33 = n32 * m1;
+          const int j = N * N - 1 - i;
+          const float4 pb = pw[(ty + j % N) * PW + tx + j / N];  // the mirrored sample's
+          const float q32 = pb.x, n32 = pb.y, q33 = pb.z, n33 = pb.w;
           // mirrored tap uses wg.wzyx
           hi.x += q32 * wg.w; hi.y += q32 * wg.z; hi.z += q32 * wg.y; hi.w += q32 * wg.x;
           lo.x += n32 * wg.w; lo.y += n32 * wg.z; lo.z += n32 * wg.y; lo.w += n32 * wg.x;
