@@ -20,14 +20,19 @@ namespace mpx {
 
 constexpr int RT_X = 64;
 constexpr int RT_Y = 4;
+// Full RAVU: 64 x 8 tiles on 512-thread blocks (one site per thread).  Their halo is up to
+// 9 rows, so the taller tile stages about a third fewer samples per site than 64 x 4 would
+// (r4 lattice: 73 x 17 instead of 73 x 13 per 2x as many sites).
+constexpr int RF_Y = 8;
+constexpr int RF_T = RT_X * RF_Y;
 
 
 // Stage P planes of a frame into LDS: tile[c][r][col] = plane c at
 // (clamp(ox + col), clamp(oy + r)), PH x PW per plane; all loads in flight at once.
-template <int P, int PH, int PW>
+template <int P, int PH, int PW, int NT = 256>
 __device__ __forceinline__ void stage_planes(float* tile, const void* base, int dt, float in_max, float in_rmax,
                                              int64_t pitch, int64_t cstride, int W, int H, int ox, int oy) {
-  HaloTile<P, PH, PW, 256> h;
+  HaloTile<P, PH, PW, NT> h;
   h.issue(base, dt, pitch, cstride, W, H, ox, oy);
   h.commit(tile, dt, in_max, in_rmax);
 }
@@ -219,17 +224,17 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
 
 // step1: int11 from the N = 2R window at offsets -(R-1)..R (ravu-r3.hook:22-146).
 template <int R, int CH>
-__global__ __launch_bounds__(256) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
+__global__ __launch_bounds__(RF_T) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
                                                          int64_t int11_frame, int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
-  constexpr int PW = RT_X + N - 1, PH = RT_Y + N - 1, PN = PW * PH;
+  constexpr int PW = RT_X + N - 1, PH = RF_Y + N - 1, PN = PW * PH;
   __shared__ float tile[staged_planes<CH>() * PN];
-  const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
+  const FrameTile ft = frame_tile<RT_X, RF_Y>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
+  stage_planes<P, PH, PW, RF_T>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
                   y0 - (R - 1));
   __syncthreads();
   if constexpr (CH == 2) {
@@ -266,21 +271,21 @@ __device__ __host__ constexpr int lat2y(int X, int Y) { return STEP == 2 ? Y - X
 // 2y+1 = (int01, int11) pairs (ravu-r3.hook:426-437), so both kernels store contiguous
 // pairs and each holds one prediction's registers only.
 template <int R, int CH, int STEP>
-__global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* __restrict__ lut,
+__global__ __launch_bounds__(RF_T) void ravu_lattice_kernel(Io io, const __half* __restrict__ lut,
                                                            const float* __restrict__ int11, int64_t int11_frame,
                                                            int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
-  constexpr int PW = RT_X + 2 * R + 1, PH = RT_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
+  constexpr int PW = RT_X + 2 * R + 1, PH = RF_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
   __shared__ float hk[staged_planes<CH>() * PN];   // input (HOOKED)
   __shared__ float i11[staged_planes<CH>() * PN];  // ravu_int11
-  const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
+  const FrameTile ft = frame_tile<RT_X, RF_Y>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
   const int64_t plane = static_cast<int64_t>(ft.W) * ft.H;
   {
-    HaloTile<P, PH, PW, 256> a, b;  // both tiles' loads in flight together
+    HaloTile<P, PH, PW, RF_T> a, b;  // both tiles' loads in flight together
     a.issue(in, io.in_dt, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - R, y0 - R);
     b.issue(int11 + f * int11_frame, DT_F32, ft.W, plane, ft.W, ft.H, x0 - R, y0 - R);
     a.commit(hk, io.in_dt, io.in_max, io.in_rmax);
@@ -323,17 +328,17 @@ __global__ __launch_bounds__(256) void ravu_lattice_kernel(Io io, const __half* 
 hipError_t launch_ravu(int r, int channels, const Io& io_, const __half* lut, float* int11, int64_t int11_frame,
                        hipStream_t st) {
   Io io = io_;
-  const int total = launch_tiles<RT_X, RT_Y>(io);
+  const int total = launch_tiles<RT_X, RF_Y>(io);
   dim3 grid(total);
 #define MPX_RAVU(R, CH)                                                                                        \
   if (r == R && channels == CH) {                                                                              \
-    hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, int11, int11_frame, total);       \
+    hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(RF_T), 0, st, io, lut, int11, int11_frame, total);       \
     hipError_t e = hipGetLastError();                                                                          \
     if (e != hipSuccess) return e;                                                                             \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2>), grid, dim3(256), 0, st, io, lut, int11, int11_frame, total);  \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2>), grid, dim3(RF_T), 0, st, io, lut, int11, int11_frame, total);  \
     e = hipGetLastError();                                                                                     \
     if (e != hipSuccess) return e;                                                                             \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3>), grid, dim3(256), 0, st, io, lut, int11, int11_frame, total);  \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3>), grid, dim3(RF_T), 0, st, io, lut, int11, int11_frame, total);  \
     return hipGetLastError();                                                                                  \
   }
   MPX_RAVU(2, 0) MPX_RAVU(2, 1) MPX_RAVU(2, 2) MPX_RAVU(3, 0) MPX_RAVU(3, 1) MPX_RAVU(3, 2) MPX_RAVU(4, 0)
