@@ -159,14 +159,8 @@ def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
         offered = {"p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
                    "mpx_s": round(total_px / (t_end - t0) / 1e6, 1), "frames": n,
                    "achieved_fps": round(n / (t_end - t0), 1)}
-        # (b) saturated: everything queued at once
-        t0 = time.perf_counter()
-        handles = [srv.submit(v, x, precision=prec[v]) for v, x in reqs]
-        for r in handles:
-            r.wait(300)
-            r.out = None
-        t_end = max(r.t_done for r in handles)
-        sat = total_px / (t_end - t0) / 1e6
+        # (b) saturated: everything queued at once, at steady state
+        sat = total_px / _burst(srv, reqs, prec) / 1e6
         launches = srv.launches
     finally:
         srv.close()
@@ -187,6 +181,21 @@ def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
             "speedup_vs_one_frame_per_launch_async": round(sat / single_async, 2),
             "speedup_vs_one_frame_per_launch_plain_fp32_sync": round(sat / plain_sync, 2),
             "launches": launches, "small_frames": small}
+
+
+def _burst(srv, reqs, prec):
+    """Seconds for the loop to finish every request queued at once, at steady state: the
+    same burst runs once untimed first, which grows the allocator's pool to the burst's
+    outputs in flight (admitted up to the loop's output budget; a loop that has been
+    serving reuses that memory), then once timed."""
+    for rep in range(2):
+        t0 = time.perf_counter()
+        handles = [srv.submit(v, x, precision=prec[v]) for v, x in reqs]
+        for r in handles:
+            r.wait(300)
+            r.out = None
+        dt = max(r.t_done for r in handles) - t0
+    return dt
 
 
 def _one_per_launch(pool, reqs, prec, total_px, sync):
@@ -232,12 +241,7 @@ def _small_frames(prec, n=2000, w=640, h=360):
         for key, frames in pool.items():
             srv.submit(key[0], frames[0], precision=prec[key[0]]).wait(120)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        hs = [srv.submit(v, x, precision=prec[v]) for v, x in reqs]
-        for r in hs:
-            r.wait(300)
-            r.out = None
-        sat = total_px / (max(r.t_done for r in hs) - t0) / 1e6
+        sat = total_px / _burst(srv, reqs, prec) / 1e6
     finally:
         srv.close()
     single = _one_per_launch(pool, reqs, prec, total_px, sync=True)
