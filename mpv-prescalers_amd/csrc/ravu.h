@@ -11,14 +11,13 @@ struct mpx_plan;
 
 namespace mpx {
 
-// Per-plan phase tables of ravu-zoom (phases repeat along each axis for rational ratios).
+// Per-plan table of ravu-zoom: the output phase repeats along each axis with period
+// out / gcd(in, out), so for small periods the bilinear LUT fetches are interpolated once per
+// (bucket row, phase pair) at plan creation instead of once per output pixel.
 struct ZoomPlan {
-  float* wtab = nullptr;       // device: pre-interpolated weights per (bucket, phase pair)
+  float* wtab = nullptr;       // device: pre-interpolated weights (nullptr: per-pixel)
   size_t wtab_bytes = 0;
-  int32_t* xphase = nullptr;   // device: per output column, phase id
-  int32_t* yphase = nullptr;   // device: per output row, phase id
-  size_t idx_bytes = 0;
-  int nphx = 0, nphy = 0;      // distinct phases per axis (0 = per-pixel interpolation)
+  int nphx = 0, nphy = 0;      // phase periods along x and y
 };
 
 int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w, int out_h);

@@ -17,6 +17,9 @@
 #include "common.h"
 #include "launch.h"
 #include "ravu.h"
+#include "internal.h"
+
+#include <numeric>
 
 namespace mpx {
 
@@ -47,8 +50,12 @@ __device__ __forceinline__ Phase phase_of(int o, int in, int out) {
 // FILTER LINEAR fetch of group g at texel coordinates (9g + tx, 9*row + ty), tx, ty in
 // [0, 8]; the integer row base is kept apart so the phase fraction keeps full FP32
 // precision.  Rows clamp to the texture (row can be -1 or past the last bucket).
+// The arithmetic is spelled out (explicit fmaf, no contraction) so that every kernel that
+// inlines it rounds the same way: the plan-time weight table and the per-pixel fetch agree
+// bit for bit.
 __device__ __forceinline__ float4 lut_bilinear(const __half* lut, int width, int rows, int g, float tx, int row,
                                                float ty) {
+#pragma clang fp contract(off)
   const float fx0 = floorf(tx), fy0 = floorf(ty);
   const float ax = tx - fx0, ay = ty - fy0;
   const int x0 = 9 * g + static_cast<int>(fx0);
@@ -61,10 +68,10 @@ __device__ __forceinline__ float4 lut_bilinear(const __half* lut, int width, int
   const float4 v11 = load_lut4(lut, static_cast<int64_t>(y1) * width + x1);
   const float bx = 1.f - ax, by = 1.f - ay;
   float4 r;
-  r.x = (v00.x * bx + v01.x * ax) * by + (v10.x * bx + v11.x * ax) * ay;
-  r.y = (v00.y * bx + v01.y * ax) * by + (v10.y * bx + v11.y * ax) * ay;
-  r.z = (v00.z * bx + v01.z * ax) * by + (v10.z * bx + v11.z * ax) * ay;
-  r.w = (v00.w * bx + v01.w * ax) * by + (v10.w * bx + v11.w * ax) * ay;
+  r.x = fmaf(fmaf(v00.x, bx, v01.x * ax), by, fmaf(v10.x, bx, v11.x * ax) * ay);
+  r.y = fmaf(fmaf(v00.y, bx, v01.y * ax), by, fmaf(v10.y, bx, v11.y * ax) * ay);
+  r.z = fmaf(fmaf(v00.z, bx, v01.z * ax), by, fmaf(v10.z, bx, v11.z * ax) * ay);
+  r.w = fmaf(fmaf(v00.w, bx, v01.w * ax), by, fmaf(v10.w, bx, v11.w * ax) * ay);
   return r;
 }
 
@@ -82,9 +89,48 @@ __device__ __forceinline__ double zluma_d(const float* t, int o, int n) {
   return static_cast<double>(t[o]);
 }
 
+// Pre-interpolated weights.  The phase of output column o repeats with period
+// out / gcd(in, out) (2 at 2x, 3 at 1.5x), so for such plans every (bucket row, phase pair)
+// is interpolated once, at plan creation, by the same lut_bilinear in the same operations:
+// table entry (row, py, px, g) holds the group's float4s [wf, wi (, af, ai)].  Bucket rows
+// below 0 fetch the same texels as row -1 and rows past the last bucket the same as
+// bucket 288 (the texel rows clamp), so rows -1..288 cover every bucket value.
+constexpr int kZoomRowLo = -1, kZoomRows = 290;
+constexpr int kZoomMaxPhasePairs = 64;
+
+template <int R, bool AR>
+__global__ __launch_bounds__(256) void zoom_table_kernel(const __half* __restrict__ lut,
+                                                         const __half* __restrict__ lut_ar, float4* wtab, int in_w,
+                                                         int in_h, int out_w, int out_h, int nphx, int nphy) {
+  constexpr int N = 2 * R;
+  constexpr int GROUPS = (N * N / 2 + 3) / 4;
+  constexpr int WIDTH = 9 * GROUPS, ROWS = 2592;
+  constexpr int NT = AR ? 4 : 2;
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= kZoomRows * nphy * nphx * GROUPS) return;
+  const int g = i % GROUPS;
+  int t = i / GROUPS;
+  const int px = t % nphx;
+  t /= nphx;
+  const int py = t % nphy;
+  const int row = t / nphy + kZoomRowLo;
+  const Phase phx = phase_of(px, in_w, out_w), phy = phase_of(py, in_h, out_h);
+  const float txf = 8.f * phx.f, txi = 8.f * (1.f - phx.f);
+  const float tyf = 8.f * phy.f, tyi = 8.f * (1.f - phy.f);
+  float4* e = wtab + static_cast<int64_t>(i) * NT;
+  e[0] = lut_bilinear(lut, WIDTH, ROWS, g, txf, row, tyf);
+  e[1] = lut_bilinear(lut, WIDTH, ROWS, g, txi, row, tyi);
+  if (AR) {
+    e[2] = lut_bilinear(lut_ar, WIDTH, ROWS, g, txf, row, tyf);
+    e[3] = lut_bilinear(lut_ar, WIDTH, ROWS, g, txi, row, tyi);
+  }
+}
+
 template <int R, bool AR, int CH>
 __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
-                                                        const __half* __restrict__ lut_ar, int out_w, int out_h) {
+                                                        const __half* __restrict__ lut_ar,
+                                                        const float4* __restrict__ wtab, int nphx, int nphy,
+                                                        int out_w, int out_h) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
   constexpr int HALF = N * N / 2;
@@ -143,6 +189,13 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   auto si = [&](int c, int i) { return c * PN + (lby + i % N) * PW + lbx + i / N; };
   const float txf = 8.f * phx.f, txi = 8.f * (1.f - phx.f);
   const float tyf = 8.f * phy.f, tyi = 8.f * (1.f - phy.f);
+  // this pixel's entries of the pre-interpolated table (when the plan has one)
+  constexpr int NT = AR ? 4 : 2;
+  const float4* wt = nullptr;
+  if (wtab) {
+    const int trow = min(max(row, kZoomRowLo), kZoomRowLo + kZoomRows - 1) - kZoomRowLo;
+    wt = wtab + static_cast<int64_t>(((trow * nphy + oy % nphy) * nphx + ox % nphx) * GROUPS) * NT;
+  }
   float res[P], hi[P], lo[P], hi2[P], lo2[P];
 #pragma unroll
   for (int c = 0; c < P; ++c) res[c] = hi[c] = lo[c] = hi2[c] = lo2[c] = 0.f;
@@ -150,13 +203,25 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   for (int g = 0; g < GROUPS; ++g) {
     // one group's operands live at a time (its sums complete before the next group)
     __builtin_amdgcn_sched_barrier(0);
-    const float4 wf4 = lut_bilinear(lut, WIDTH, ROWS, g, txf, row, tyf);
-    const float4 wi4 = lut_bilinear(lut, WIDTH, ROWS, g, txi, row, tyi);
+    float4 wf4, wi4, a4, b4;
+    if (wt) {
+      wf4 = wt[g * NT];
+      wi4 = wt[g * NT + 1];
+      if (AR) {
+        a4 = wt[g * NT + 2];
+        b4 = wt[g * NT + 3];
+      }
+    } else {
+      wf4 = lut_bilinear(lut, WIDTH, ROWS, g, txf, row, tyf);
+      wi4 = lut_bilinear(lut, WIDTH, ROWS, g, txi, row, tyi);
+      if (AR) {
+        a4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txf, row, tyf);
+        b4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txi, row, tyi);
+      }
+    }
     const float wf[4] = {wf4.x, wf4.y, wf4.z, wf4.w}, wi[4] = {wi4.x, wi4.y, wi4.z, wi4.w};
     float af[4] = {0.f, 0.f, 0.f, 0.f}, ai[4] = {0.f, 0.f, 0.f, 0.f};
     if (AR) {
-      const float4 a4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txf, row, tyf);
-      const float4 b4 = lut_bilinear(lut_ar, WIDTH, ROWS, g, txi, row, tyi);
       af[0] = a4.x; af[1] = a4.y; af[2] = a4.z; af[3] = a4.w;
       ai[0] = b4.x; ai[1] = b4.y; ai[2] = b4.z; ai[3] = b4.w;
     }
@@ -204,11 +269,11 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
 
 hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
                             const ZoomPlan& zp, int out_w, int out_h, hipStream_t st) {
-  (void)zp;
   dim3 grid(((out_w + ZT_X - 1) / ZT_X) * ((out_h + ZT_Y - 1) / ZT_Y) * io.batch);
 #define MPX_ZOOM(R, A, CH)                                                                             \
   if (r == R && ar == A && channels == CH) {                                                           \
-    hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH>), grid, dim3(256), 0, st, io, lut, lut_ar, out_w, out_h); \
+    hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH>), grid, dim3(256), 0, st, io, lut, lut_ar,                 \
+                       reinterpret_cast<const float4*>(zp.wtab), zp.nphx, zp.nphy, out_w, out_h);        \
     return hipGetLastError();                                                                          \
   }
   MPX_ZOOM(2, false, 0) MPX_ZOOM(2, false, 1) MPX_ZOOM(2, false, 2) MPX_ZOOM(3, false, 0) MPX_ZOOM(3, false, 1)
@@ -218,7 +283,54 @@ hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __
   return hipErrorInvalidValue;
 }
 
-int zoom_plan_create(mpx_ctx*, mpx_plan*, int, int, int, int) { return 0; }
-void zoom_plan_destroy(mpx_ctx*, mpx_plan*) {}
+int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w, int out_h) {
+  ZoomPlan& zp = plan->zoom;
+  zp = ZoomPlan{};
+  const int nphx = out_w / std::gcd(in_w, out_w), nphy = out_h / std::gcd(in_h, out_h);
+  if (static_cast<int64_t>(nphx) * nphy > kZoomMaxPhasePairs) return MPX_OK;  // per-pixel interpolation
+  const int r = plan->v->radius;
+  const bool ar = plan->v->ar != 0;
+  const int groups = (2 * r * 2 * r / 2 + 3) / 4;
+  const int entries = kZoomRows * nphy * nphx * groups;
+  const size_t bytes = static_cast<size_t>(entries) * (ar ? 4 : 2) * sizeof(float4);
+  void* p = nullptr;
+  if (ctx->alloc(bytes, &p)) return MPX_OK;  // over the budget: per-pixel interpolation
+  mpx_host::DeviceScope ds;
+  int rc;
+  if ((rc = ds.enter(ctx->device))) {
+    ctx->release(p, bytes);
+    return rc;
+  }
+  const dim3 grid((entries + 255) / 256);
+  float4* tab = static_cast<float4*>(p);
+  if (r == 2 && !ar)
+    hipLaunchKernelGGL((zoom_table_kernel<2, false>), grid, dim3(256), 0, nullptr, plan->lut, plan->lut_ar, tab, in_w,
+                       in_h, out_w, out_h, nphx, nphy);
+  else if (r == 2)
+    hipLaunchKernelGGL((zoom_table_kernel<2, true>), grid, dim3(256), 0, nullptr, plan->lut, plan->lut_ar, tab, in_w,
+                       in_h, out_w, out_h, nphx, nphy);
+  else if (!ar)
+    hipLaunchKernelGGL((zoom_table_kernel<3, false>), grid, dim3(256), 0, nullptr, plan->lut, plan->lut_ar, tab, in_w,
+                       in_h, out_w, out_h, nphx, nphy);
+  else
+    hipLaunchKernelGGL((zoom_table_kernel<3, true>), grid, dim3(256), 0, nullptr, plan->lut, plan->lut_ar, tab, in_w,
+                       in_h, out_w, out_h, nphx, nphy);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    ctx->release(p, bytes);
+    return mpx_host::fail(MPX_ERR_HIP, std::string("ravu-zoom weight table: ") + hipGetErrorString(e));
+  }
+  zp.wtab = static_cast<float*>(p);
+  zp.wtab_bytes = bytes;
+  zp.nphx = nphx;
+  zp.nphy = nphy;
+  return MPX_OK;
+}
+
+void zoom_plan_destroy(mpx_ctx* ctx, mpx_plan* plan) {
+  ctx->release(plan->zoom.wtab, plan->zoom.wtab_bytes);
+  plan->zoom = ZoomPlan{};
+}
 
 }  // namespace mpx

@@ -173,3 +173,28 @@ def test_ravu_integer_input_equals_float(name, target, bits):
         xf = torch.from_numpy(u16[i].astype(np.float32) / np.float32(full)).cuda()
         single = mpx.upscale(name, xf, target=target)
         assert torch.equal(out[i], single), (name, bits, i)
+
+
+@pytest.mark.parametrize("r,ar,ch,ins,outs", [(2, True, "rgb", (128, 72), (256, 144)),
+                                              (2, False, "luma", (120, 80), (180, 120)),
+                                              (3, False, "yuv", (96, 64), (192, 128))])
+def test_ravu_zoom_weight_table_equals_per_pixel(r, ar, ch, ins, outs):
+    """Where the output phase repeats with a short period (2x: 2, 1.5x: 3), a plan takes
+    its bilinear LUT weights from a table interpolated once at plan creation.  A context
+    whose HBM budget holds the LUTs but not that table interpolates per output pixel
+    instead.  Both give the same output, bit for bit."""
+    import mpx
+    planes = 1 if ch == "luma" else 3
+    img = _frame(ins[0], ins[1], seed=70 + r, planes=planes)
+    name = "ravu-zoom%s-r%d%s" % ("-ar" if ar else "", r, _suffix(ch))
+    x = torch.from_numpy(img).cuda()
+    groups = ((2 * r) ** 2 // 2 + 3) // 4
+    lut_bytes = 2592 * 9 * groups * 4 * 2 * (2 if ar else 1)   # FP16 LUT(s) resident in the context
+    outs_ = []
+    for budget in (16 << 30, lut_bytes + (32 << 10)):       # the smallest table is 74 KB
+        ctx = mpx.Context(0, hbm_budget=budget)
+        plan = ctx.plan(name, ins[0], ins[1], target=outs)
+        outs_.append(plan.run(x).cpu())
+        plan.close()
+        ctx.close()
+    assert torch.equal(outs_[0], outs_[1])
