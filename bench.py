@@ -32,7 +32,9 @@ RAVU_SECONDARY = (("ravu-lite-ar-r3", (640, 360), None, 1, 64, "float32"),
                   ("ravu-lite-ar-r3", (1920, 1080), None, 1, 8, "float16"),
                   ("ravu-zoom-ar-r2-rgb", (1280, 720), (2560, 1440), 3, 4, "float16"),
                   ("ravu-r4-yuv", (1920, 1080), None, 3, 2, "float16"),
-                  ("ravu-3x-r3", (1920, 1080), None, 1, 2, "float16"))
+                  ("ravu-3x-r3", (1920, 1080), None, 1, 2, "float16"),
+                  # 1.5x, radius 3 (the r3 anti-ringing table is not shipped)
+                  ("ravu-zoom-r3-rgb", (1920, 1080), (2880, 1620), 3, 1, "float16"))
 
 
 def nnedi3_flop_per_px(name):
@@ -403,10 +405,10 @@ def main():
         f10 = torch.from_numpy(np.stack([make_plane(W, H, seed=20 + i, bits=10) for i in range(b)])
                                .astype(np.float16)).to(dev)
         o10 = torch.empty((b, 2 * H, 2 * W), dtype=torch.float16, device=dev)
-        for p in ("fp16", "fp32"):
+        for p in ("fp16", "bf16", "fp8", "fp32"):
             pl = ctx.plan("nnedi3-nns256-win8x6", W, H, in_dtype=torch.float16, out_dtype=torch.float16,
                           precision=p, max_batch=b)
-            n = 5 if p == "fp16" else 2
+            n = 2 if p == "fp32" else 5
             pl.tune(f10, o10, iters=2)
             t, wall = timed(lambda: pl.run(f10, out=o10), n, 1, torch)
             fps = b * n / wall
@@ -421,7 +423,9 @@ def main():
             t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
             opx = pl.info["out_w"] * pl.info["out_h"]
             io_bytes = (iw * ih + opx) * planes * (4 if dt == torch.float32 else 2)
-            secondary["%s %dx%d b%d" % (name, iw, ih, b)] = {
+            key = "%s %dx%d b%d" % (name, iw, ih, b) if tgt is None or name.endswith("ar-r2-rgb") else \
+                "%s %dx%d->%dx%d b%d" % (name, iw, ih, tgt[0], tgt[1], b)
+            secondary[key] = {
                 "mpx_s": round(b * 10 * opx / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
                 "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
                 "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4)}
