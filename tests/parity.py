@@ -6,9 +6,8 @@ import sys
 
 import numpy as np
 
-from conftest import ROOT
-
-REPORT = os.path.join(ROOT, "gpurun_out", "parity_report.jsonl")
+# JSON-lines file the parity records are appended to (unset: printed only)
+REPORT = os.environ.get("MPX_PARITY_REPORT")
 
 
 def psnr(err):
@@ -17,16 +16,17 @@ def psnr(err):
 
 
 def report(case, **fields):
-    """Append one parity record (visible in the GPU run's output folder) and print it."""
+    """Print one parity record, and append it to $MPX_PARITY_REPORT when that is set."""
     rec = dict(case=case, **fields)
     line = json.dumps(rec, sort_keys=True, default=float)
     print("PARITY " + line, flush=True)
-    try:
-        os.makedirs(os.path.dirname(REPORT), exist_ok=True)
-        with open(REPORT, "a") as f:
-            f.write(line + "\n")
-    except OSError:
-        pass
+    if REPORT:
+        try:
+            os.makedirs(os.path.dirname(os.path.abspath(REPORT)), exist_ok=True)
+            with open(REPORT, "a") as f:
+                f.write(line + "\n")
+        except OSError:
+            pass
     sys.stdout.flush()
 
 
