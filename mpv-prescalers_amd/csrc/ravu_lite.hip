@@ -61,15 +61,25 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
 
-  // window sample i = X*N + Y at offset (X - HALO, Y - HALO)
-  float s[N * N];
+  // window sample i = X*N + Y at offset (X - HALO, Y - HALO).  r <= 3 hold the window in
+  // registers from the hash to the taps; r4 (49 samples) reads it from LDS where it is
+  // used, with the lean tensor forms, so the rare FP64 re-decision does not stack its
+  // registers on a live window (82 -> 61 VGPRs, 5 -> 8 waves per SIMD)
+  constexpr bool FROM_LDS = R >= 4;
+  auto lds = [&](int i) { return tile[(ty + i % N) * PW + tx + i / N]; };
+  float s[FROM_LDS ? 1 : N * N];
+  if constexpr (!FROM_LDS) {
 #pragma unroll
-  for (int X = 0; X < N; ++X)
+    for (int X = 0; X < N; ++X)
 #pragma unroll
-    for (int Y = 0; Y < N; ++Y) s[X * N + Y] = tile[(ty + Y) * PW + tx + X];
-
-  const Bucket bk = ravu_hash<N, false, RULE_LITE>(
-      [&](int i) { return s[i]; }, [&](int i) { return static_cast<double>(s[i]); });
+      for (int Y = 0; Y < N; ++Y) s[X * N + Y] = tile[(ty + Y) * PW + tx + X];
+  }
+  auto sample = [&](int i) {
+    if constexpr (FROM_LDS) return lds(i);
+    else return s[i];
+  };
+  const Bucket bk = ravu_hash<N, false, RULE_LITE, FROM_LDS>(
+      [&](int i) { return sample(i); }, [&](int i) { return static_cast<double>(sample(i)); });
   const int row = bucket_row<RULE_LITE>(bk, 288);
   if (io.buckets)
     io.buckets[frame * io.bucket_bstride + static_cast<int64_t>(y) * ft.W + x] = row;
@@ -89,13 +99,13 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
     if (i < C) tn = lraw[i + 1];
     const __half2 h0 = bits_h2(tc.x), h1 = bits_h2(tc.y);
     const float4 w = make_float4(__low2float(h0), __high2float(h0), __low2float(h1), __high2float(h1));
-    const float li = s[i];
+    const float li = sample(i);
     res.x = fmaf(li, w.x, res.x);
     res.y = fmaf(li, w.y, res.y);
     res.z = fmaf(li, w.z, res.z);
     res.w = fmaf(li, w.w, res.w);
     if (i < C) {  // the mirrored sample takes the texel reversed (w.wzyx)
-      const float lj = s[N * N - 1 - i];
+      const float lj = sample(N * N - 1 - i);
       res.x = fmaf(lj, w.w, res.x);
       res.y = fmaf(lj, w.z, res.y);
       res.z = fmaf(lj, w.y, res.z);
