@@ -164,7 +164,9 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   for (int i = threadIdx.x; i < nbx * nby; i += blockDim.x) {
     const int by = i / nbx, bx = i - by * nbx;
     auto off = [&](int s) { return (by + s % N) * PW + bx + s / N; };
-    const Bucket bk = ravu_hash<N, true, RULE_LITE>([&](int s) { return zluma_f<CH>(tile, off(s), PN); },
+    // r3 (6x6 windows): lean tensor forms, so the rare FP64 re-decision does not set the
+    // kernel's register count (r3 rgb: 216 -> 137 VGPRs); r2 keeps the fused forms
+    const Bucket bk = ravu_hash<N, true, RULE_LITE, (R >= 3)>([&](int s) { return zluma_f<CH>(tile, off(s), PN); },
                                                     [&](int s) { return zluma_d<CH>(tile, off(s), PN); });
     bucket[by * ZB_X + bx] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
   }
