@@ -89,6 +89,20 @@ __device__ __forceinline__ double zluma_d(const float* t, int o, int n) {
   return static_cast<double>(t[o]);
 }
 
+// FP64 re-decision of one base pixel's hash, out of line (the kernels without anti-ringing):
+// it runs for a small fraction of sites, and inlined its registers (the double atan2 alone
+// takes 56) would set the occupancy of every wave (zoom-r2: 96 -> 51 VGPRs, r3 rgb: 137 ->
+// 61).  Its register budget follows its callers' (6 waves per SIMD, below), so the lean
+// tensor form keeps it from spilling.  `t` is the staged tile, `o` the window's first sample.
+template <int N, int CH, int PW, int PN>
+__device__ __attribute__((noinline)) int zoom_redecide(const float* t, int o) {
+  double a, b, d;
+  structure_tensor<N, true, double, true>([&](int s) { return zluma_d<CH>(t, o + (s % N) * PW + s / N, PN); },
+                                          a, b, d);
+  const Bucket bk = hash_reference<RULE_LITE, double>(a, b, d);
+  return (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+}
+
 // Pre-interpolated weights.  The phase of output column o repeats with period
 // out / gcd(in, out) (2 at 2x, 3 at 1.5x), so for such plans every (bucket row, phase pair)
 // is interpolated once, at plan creation, by the same lut_bilinear in the same operations:
@@ -126,8 +140,10 @@ __global__ __launch_bounds__(256) void zoom_table_kernel(const __half* __restric
   }
 }
 
+// (kernels without anti-ringing: at least 6 waves per SIMD, a register budget their
+// out-of-line re-decision inherits)
 template <int R, bool AR, int CH>
-__global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6))) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
                                                         const __half* __restrict__ lut_ar,
                                                         const float4* __restrict__ wtab, int nphx, int nphy,
                                                         int out_w, int out_h) {
@@ -164,11 +180,20 @@ __global__ __launch_bounds__(256) void ravu_zoom_kernel(Io io, const __half* __r
   for (int i = threadIdx.x; i < nbx * nby; i += blockDim.x) {
     const int by = i / nbx, bx = i - by * nbx;
     auto off = [&](int s) { return (by + s % N) * PW + bx + s / N; };
-    // r3 (6x6 windows): lean tensor forms, so the rare FP64 re-decision does not set the
-    // kernel's register count (r3 rgb: 216 -> 137 VGPRs); r2 keeps the fused forms
-    const Bucket bk = ravu_hash<N, true, RULE_LITE, (R >= 3)>([&](int s) { return zluma_f<CH>(tile, off(s), PN); },
-                                                    [&](int s) { return zluma_d<CH>(tile, off(s), PN); });
-    bucket[by * ZB_X + bx] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+    if constexpr (AR) {
+      // the anti-ringing tap loop sets these kernels' register count: the re-decision stays
+      // inline, in the lean tensor forms for r3 (r3 rgb: 216 -> 123 VGPRs)
+      const Bucket bk = ravu_hash<N, true, RULE_LITE, (R >= 3)>([&](int s) { return zluma_f<CH>(tile, off(s), PN); },
+                                                      [&](int s) { return zluma_d<CH>(tile, off(s), PN); });
+      bucket[by * ZB_X + bx] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+    } else {
+      float ta, tb, td;
+      structure_tensor<N, true, float, (R >= 3)>([&](int s) { return zluma_f<CH>(tile, off(s), PN); }, ta, tb, td);
+      bool near = false;
+      const Bucket bk = hash_fast<RULE_LITE>(ta, tb, td, &near);
+      bucket[by * ZB_X + bx] = near ? zoom_redecide<N, CH, PW, PN>(tile, off(0))
+                                    : (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+    }
   }
   if constexpr (AR) {
     for (int i = threadIdx.x; i < P * PN; i += blockDim.x) {
