@@ -71,6 +71,28 @@ __device__ __forceinline__ double luma_d(const float* t, int o, int n) {
   return static_cast<double>(t[o]);
 }
 
+// Hash of the site whose N x N window starts at staged-tile offset o (sample i at
+// o + (i % N) * PW + i / N), with the FP64 re-decision out of line: it is rare, and inlined
+// its registers (the double atan2 alone takes 56) would set the occupancy of every wave
+// (r4 rgb step1: 96 -> 51 VGPRs).  Its register budget follows its callers' (the kernel asks
+// for 6 waves per SIMD), so it takes the lean tensor form to stay out of scratch.  Used
+// where it pays (full-RAVU step1); ravu-3x measured 2-3 % slower with it.
+template <int N, bool FIVE, int RULE, int CH, int PW, int PN>
+__device__ __attribute__((noinline)) Bucket redecide_site(const float* t, int o) {
+  double a, b, d;
+  structure_tensor<N, FIVE, double, true>([&](int s) { return luma_d<CH>(t, o + (s % N) * PW + s / N, PN); }, a, b,
+                                          d);
+  return hash_reference<RULE, double>(a, b, d);
+}
+template <int N, bool FIVE, int RULE, int CH, int PW, int PN, bool LEAN>
+__device__ __forceinline__ Bucket hash_site(const float* t, int o) {
+  float a, b, d;
+  structure_tensor<N, FIVE, float, LEAN>([&](int s) { return luma_f<CH>(t, o + (s % N) * PW + s / N, PN); }, a, b, d);
+  bool near = false;
+  const Bucket bk = hash_fast<RULE>(a, b, d, &near);
+  return near ? redecide_site<N, FIVE, RULE, CH, PW, PN>(t, o) : bk;
+}
+
 // ----------------------------------------------------------------------------- RAVU-3x
 
 template <int R, int CH>
@@ -224,7 +246,7 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
 
 // step1: int11 from the N = 2R window at offsets -(R-1)..R (ravu-r3.hook:22-146).
 template <int R, int CH>
-__global__ __launch_bounds__(RF_T) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
+__global__ __launch_bounds__(RF_T) __attribute__((amdgpu_waves_per_eu(6))) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
                                                          int64_t int11_frame, int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
@@ -245,8 +267,7 @@ __global__ __launch_bounds__(RF_T) void ravu_step1_kernel(Io io, const __half* _
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
   auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };
-  const Bucket bk = ravu_hash<N, true, RULE_LOG2, true>([&](int i) { return luma_f<CH>(tile, off(i), PN); },
-                                                  [&](int i) { return luma_d<CH>(tile, off(i), PN); });
+  const Bucket bk = hash_site<N, true, RULE_LOG2, CH, PW, PN, true>(tile, off(0));
   const int row = bucket_row<RULE_LOG2>(bk, 648);
   if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(y) * ft.W + x] = row;
   float res[P];
