@@ -94,7 +94,8 @@ __device__ __forceinline__ double zluma_d(const float* t, int o, int n) {
 // takes 56) would set the occupancy of every wave (zoom-r2: 96 -> 51 VGPRs, r3 rgb: 137 ->
 // 61).  Its register budget follows its callers' (6 waves per SIMD, below), so the lean
 // tensor form keeps it from spilling.  `t` is the staged tile, `o` the window's first sample.
-template <int N, int CH, int PW, int PN>
+// (one instance per calling kernel, TAB, so each inherits its own caller's budget)
+template <int N, int CH, int PW, int PN, bool TAB>
 __device__ __attribute__((noinline)) int zoom_redecide(const float* t, int o) {
   double a, b, d;
   structure_tensor<N, true, double, true>([&](int s) { return zluma_d<CH>(t, o + (s % N) * PW + s / N, PN); },
@@ -141,8 +142,9 @@ __global__ __launch_bounds__(256) void zoom_table_kernel(const __half* __restric
 }
 
 // (kernels without anti-ringing: at least 6 waves per SIMD, a register budget their
-// out-of-line re-decision inherits)
-template <int R, bool AR, int CH>
+// out-of-line re-decision inherits.)  TAB: the plan has the pre-interpolated weight table;
+// separate instances keep the per-pixel bilinear fetch's registers out of the table path.
+template <int R, bool AR, int CH, bool TAB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6))) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
                                                         const __half* __restrict__ lut_ar,
                                                         const float4* __restrict__ wtab, int nphx, int nphy,
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
       structure_tensor<N, true, float, (R >= 3)>([&](int s) { return zluma_f<CH>(tile, off(s), PN); }, ta, tb, td);
       bool near = false;
       const Bucket bk = hash_fast<RULE_LITE>(ta, tb, td, &near);
-      bucket[by * ZB_X + bx] = near ? zoom_redecide<N, CH, PW, PN>(tile, off(0))
+      bucket[by * ZB_X + bx] = near ? zoom_redecide<N, CH, PW, PN, TAB>(tile, off(0))
                                     : (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
     }
   }
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
   // this pixel's entries of the pre-interpolated table (when the plan has one)
   constexpr int NT = AR ? 4 : 2;
   const float4* wt = nullptr;
-  if (wtab) {
+  if constexpr (TAB) {
     const int trow = min(max(row, kZoomRowLo), kZoomRowLo + kZoomRows - 1) - kZoomRowLo;
     wt = wtab + static_cast<int64_t>(((trow * nphy + oy % nphy) * nphx + ox % nphx) * GROUPS) * NT;
   }
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
     // one group's operands live at a time (its sums complete before the next group)
     __builtin_amdgcn_sched_barrier(0);
     float4 wf4, wi4, a4, b4;
-    if (wt) {
+    if constexpr (TAB) {
       wf4 = wt[g * NT];
       wi4 = wt[g * NT + 1];
       if (AR) {
@@ -299,8 +301,12 @@ hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __
   dim3 grid(((out_w + ZT_X - 1) / ZT_X) * ((out_h + ZT_Y - 1) / ZT_Y) * io.batch);
 #define MPX_ZOOM(R, A, CH)                                                                             \
   if (r == R && ar == A && channels == CH) {                                                           \
-    hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH>), grid, dim3(256), 0, st, io, lut, lut_ar,                 \
-                       reinterpret_cast<const float4*>(zp.wtab), zp.nphx, zp.nphy, out_w, out_h);        \
+    if (zp.wtab)                                                                                       \
+      hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, true>), grid, dim3(256), 0, st, io, lut, lut_ar,    \
+                         reinterpret_cast<const float4*>(zp.wtab), zp.nphx, zp.nphy, out_w, out_h);      \
+    else                                                                                               \
+      hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, false>), grid, dim3(256), 0, st, io, lut, lut_ar,   \
+                         nullptr, 0, 0, out_w, out_h);                                                   \
     return hipGetLastError();                                                                          \
   }
   MPX_ZOOM(2, false, 0) MPX_ZOOM(2, false, 1) MPX_ZOOM(2, false, 2) MPX_ZOOM(3, false, 0) MPX_ZOOM(3, false, 1)
