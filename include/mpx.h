@@ -82,7 +82,7 @@ typedef struct mpx_io {
   int64_t out_pitch, out_cstride, out_bstride;
   int32_t batch;
   int32_t* buckets;     /* optional (may be NULL): per-site LUT row ids, int32, see
-                           mpx_plan_bucket_layout; for parity reporting               */
+                           mpx_plan_bucket_count; for parity reporting               */
 } mpx_io;
 
 /* ---- host-only queries (no GPU needed) -------------------------------------------- */
@@ -97,6 +97,16 @@ const char* mpx_variant_name(int i);
  * 349-352; ravu-zoom-r2.hook:18-21; compute/ravu-3x-r3.hook:18-20). */
 int mpx_query(const char* name, int in_w, int in_h, int target_w, int target_h,
               mpx_variant_info* info);
+
+/* Check every weight file variant `name` loads from `weights_dir` (NULL = the default
+ * directory) without a device: MPXW header, shape against the shape the kernels index
+ * with, file size, and for the tables the reference ships the SHA-256 of the data
+ * against the value recorded when they were extracted.  MPX_ERR_IO names the first bad
+ * file; MPX_ERR_UNAVAILABLE when an unshipped table (ravu-zoom-ar-r3) is not supplied.
+ * mpx_plan_create makes the same checks before it uploads anything.  Replaces mpv's
+ * parsing of the `//!TEXTURE` block's SIZE / FORMAT lines and its hex data (e.g.
+ * ravu-zoom-ar-r2-rgb.hook:226-234: SIZE 18 2592, FORMAT rgba16f). */
+int mpx_weights_verify(const char* weights_dir, const char* name);
 
 /* ---- context ---------------------------------------------------------------------- */
 

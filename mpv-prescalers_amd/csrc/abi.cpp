@@ -161,6 +161,24 @@ int mpx_query(const char* name, int in_w, int in_h, int target_w, int target_h, 
   return MPX_OK;
 }
 
+int mpx_weights_verify(const char* weights_dir, const char* name) {
+  const Variant* v = find_variant(name);
+  if (!v) return fail(MPX_ERR_UNKNOWN_VARIANT, std::string("unknown variant '") + (name ? name : "(null)") + "'");
+  const std::string dir = weights_dir ? weights_dir : default_weights_dir();
+  for (const std::string& t : mpx_host::variant_tables(*v)) {
+    if (!v->available && t.find("_ar") != std::string::npos) {
+      FILE* f = std::fopen((dir + "/" + t + ".bin").c_str(), "rb");
+      if (!f)
+        return fail(MPX_ERR_UNAVAILABLE, std::string(name) + ": " + t + ".bin is not shipped by the reference");
+      std::fclose(f);
+    }
+    std::vector<float> data;
+    std::vector<uint32_t> dims;
+    if (int rc = mpx_host::load_table(dir, t, data, dims)) return rc;
+  }
+  return MPX_OK;
+}
+
 int mpx_create(int device, uint64_t hbm_budget, const char* weights_dir, mpx_ctx** out) {
   if (!out) return fail(MPX_ERR_ARG, "out is NULL");
   *out = nullptr;
@@ -199,6 +217,14 @@ int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int targ
                                            ".MISSING_LARGE_BLOBS); supply ravu_zoom_lut" +
                                            std::to_string(v->radius) + "_ar.bin in the weights directory");
     std::fclose(fchk);
+  }
+  // every table the variant reads is checked (shape, size, SHA-256 of shipped data) before
+  // any of them is uploaded; resident tables were checked when they were loaded
+  for (const std::string& t : mpx_host::variant_tables(*v)) {
+    if (ctx->has(t)) continue;
+    std::vector<float> data;
+    std::vector<uint32_t> dims;
+    if (int rc = mpx_host::load_table(ctx->wdir, t, data, dims)) return rc;
   }
   if (in_w <= 0 || in_h <= 0 || target_w <= 0 || target_h <= 0 || in_w > 32768 || in_h > 32768 ||
       target_w > 65536 || target_h > 65536)
@@ -286,9 +312,15 @@ void mpx_plan_destroy(mpx_plan* plan) {
   delete plan;
 }
 
-static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st);
+static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* scratch);
 
 int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
+  return mpx_host::run_io(plan, io, hip_stream, plan ? plan->scratch : nullptr);
+}
+
+}  // extern "C"
+
+int mpx_host::run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* scratch) {
   if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
   if (!io->in || !io->out) return fail(MPX_ERR_ARG, "in/out is NULL");
   if (io->batch < 1 || io->batch > plan->max_batch)
@@ -326,8 +358,10 @@ int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = io->batch;
-  return dispatch(plan, d, st);
+  return dispatch(plan, d, st, scratch);
 }
+
+extern "C" {
 
 int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* outs, int64_t in_pitch,
                    int64_t out_pitch, void* hip_stream) {
@@ -362,7 +396,7 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = n;
-  return dispatch(plan, d, static_cast<hipStream_t>(hip_stream));
+  return dispatch(plan, d, static_cast<hipStream_t>(hip_stream), plan->scratch);
 }
 
 int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* const* outs, const int* widths,
@@ -407,10 +441,10 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = n;
-  return dispatch(plan, d, static_cast<hipStream_t>(hip_stream));
+  return dispatch(plan, d, static_cast<hipStream_t>(hip_stream), plan->scratch);
 }
 
-static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st) {
+static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* scratch) {
   const mpx_variant_info& inf = plan->info;
   hipError_t e = hipSuccess;
   if (!inf.applies) {
@@ -425,15 +459,15 @@ static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st) {
         e = mpx::launch_ravu_3x(v.radius, v.channels, d, plan->lut, st);
         break;
       case MPX_FAMILY_RAVU:
-        e = mpx::launch_ravu(v.radius, v.channels, d, plan->lut, static_cast<float*>(plan->scratch),
-                             plan->scratch_frame, st);
+        e = mpx::launch_ravu(v.radius, v.channels, d, plan->lut, static_cast<float*>(scratch), plan->scratch_frame,
+                             st);
         break;
       case MPX_FAMILY_RAVU_ZOOM:
         e = mpx::launch_ravu_zoom(v.radius, v.ar != 0, v.channels, d, plan->lut, plan->lut_ar, plan->zoom,
                                   inf.out_w, inf.out_h, st);
         break;
       case MPX_FAMILY_NNEDI3:
-        e = mpx::launch_nnedi3(plan->nw, d, inf.scale_x == 2, inf.scale_y == 2, static_cast<float*>(plan->scratch),
+        e = mpx::launch_nnedi3(plan->nw, d, inf.scale_x == 2, inf.scale_y == 2, static_cast<float*>(scratch),
                                plan->scratch_frame, st);
         break;
     }

@@ -58,32 +58,28 @@ class DeviceScope {
   bool set_ = false;
 };
 
-inline bool read_weight_file(const std::string& path, std::vector<float>& data, std::vector<uint32_t>& dims) {
-  FILE* f = std::fopen(path.c_str(), "rb");
-  if (!f) return false;
-  char magic[4];
-  uint32_t hdr[3];
-  bool ok = std::fread(magic, 1, 4, f) == 4 && std::memcmp(magic, "MPXW", 4) == 0 &&
-            std::fread(hdr, 4, 3, f) == 3 && hdr[0] == 1 && hdr[1] == 0 && hdr[2] >= 1 && hdr[2] <= 4;
-  if (ok) {
-    dims.resize(hdr[2]);
-    ok = std::fread(dims.data(), 4, hdr[2], f) == hdr[2];
-  }
-  if (ok) {
-    size_t n = 1;
-    for (uint32_t d : dims) n *= d;
-    data.resize(n);
-    ok = std::fread(data.data(), 4, n, f) == n;
-  }
-  std::fclose(f);
-  return ok;
-}
-
 struct Variant {
   std::string name;
   int family, channels, radius, nns, win_h, ar, available;
 };
 
+// Weight files (weights.cpp).  load_table reads `dir`/`name`.bin after checking its header,
+// its shape against the shape the kernels index with, its size and, for tables the
+// reference ships, the SHA-256 of its data; MPX_ERR_IO (message set) on any mismatch,
+// before anything is allocated on the device.
+int load_table(const std::string& dir, const std::string& name, std::vector<float>& data,
+               std::vector<uint32_t>& dims);
+bool expected_dims(const std::string& name, std::vector<uint32_t>& dims);
+std::vector<std::string> variant_tables(const Variant& v);
+std::string sha256_of(const void* data, size_t len);
+
+}  // namespace mpx_host
+
+struct mpx_plan;
+namespace mpx_host {
+// mpx_run_ex with the intermediates in `scratch` (plan->scratch_frame FP32 elements per
+// frame; the caller has checked its size).
+int run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* scratch);
 }  // namespace mpx_host
 
 using mpx_host::fail;
@@ -133,8 +129,7 @@ struct mpx_ctx {
     if (it == weights.end()) {
       std::vector<float> data;
       std::vector<uint32_t> d;
-      if (!mpx_host::read_weight_file(wdir + "/" + name + ".bin", data, d))
-        return fail(MPX_ERR_IO, "cannot read weight file " + wdir + "/" + name + ".bin");
+      if (int rc = mpx_host::load_table(wdir, name, data, d)) return rc;
       std::vector<_Float16> h(data.size());
       for (size_t i = 0; i < data.size(); ++i) h[i] = static_cast<_Float16>(data[i]);
       void* p = nullptr;
@@ -151,9 +146,7 @@ struct mpx_ctx {
 
   // Host copy of a float32 weight file (NNEDI3 tables are repacked per precision).
   int host_table(const std::string& name, std::vector<float>& data, std::vector<uint32_t>& d) {
-    if (!mpx_host::read_weight_file(wdir + "/" + name + ".bin", data, d))
-      return fail(MPX_ERR_IO, "cannot read weight file " + wdir + "/" + name + ".bin");
-    return MPX_OK;
+    return mpx_host::load_table(wdir, name, data, d);
   }
 
   // Upload an already-packed device table once under `key`.

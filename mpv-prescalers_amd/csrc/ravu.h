@@ -14,10 +14,17 @@ namespace mpx {
 // Per-plan table of ravu-zoom: the output phase repeats along each axis with period
 // out / gcd(in, out), so for small periods the bilinear LUT fetches are interpolated once per
 // (bucket row, phase pair) at plan creation instead of once per output pixel.
+//
+// Every plan also holds the phase of each output column and row (int4 {base pixel, FP32
+// bits of the sub-pixel phase f, index of the phase within its period, 0}: out_w entries
+// for x, then out_h for y), computed once at plan creation in exact integer arithmetic, so
+// the kernel reads them instead of dividing per output pixel.
 struct ZoomPlan {
   float* wtab = nullptr;       // device: pre-interpolated weights (nullptr: per-pixel)
   size_t wtab_bytes = 0;
   int nphx = 0, nphy = 0;      // phase periods along x and y
+  int4* phase = nullptr;       // device: out_w + out_h phase entries
+  size_t phase_bytes = 0;
 };
 
 int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w, int out_h);

@@ -8,7 +8,9 @@
 // 32 x 8 output tile touches (into LDS), then each output pixel reads its base's bucket;
 // AR variants also tabulate the per-sample powers of the anti-ringing weights there.
 // Base and phase are computed in exact integer arithmetic:
-//   pos - 1/2 = ((2o+1)*in - out) / (2*out)  ->  base = floor, f = remainder / (2*out).
+//   pos - 1/2 = ((2o+1)*in - out) / (2*out)  ->  base = floor, f = remainder / (2*out),
+// once per output column and row at plan creation (zoom_phase_kernel); the kernel reads
+// them from that table.
 // LUT (FILTER LINEAR): group g of 4 taps at texel x = 9g + 8f, row y = 9*bucket + 8f_y
 // (LUTPOS, :21-31), bilinear between neighbouring phase texels, rows clamped to the
 // texture; taps p < n^2/2 at phase f, tap n^2-1-p at phase 1-f.  AR variants use the
@@ -141,6 +143,18 @@ __global__ __launch_bounds__(256) void zoom_table_kernel(const __half* __restric
   }
 }
 
+// Phase table of a plan: entry o < out_w is output column o, entry out_w + o row o:
+// {base, bits of f, o mod period, 0}.
+__global__ __launch_bounds__(256) void zoom_phase_kernel(int4* tab, int in_w, int in_h, int out_w, int out_h,
+                                                         int nphx, int nphy) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= out_w + out_h) return;
+  const bool x = i < out_w;
+  const int o = x ? i : i - out_w;
+  const Phase ph = x ? phase_of(o, in_w, out_w) : phase_of(o, in_h, out_h);
+  tab[i] = make_int4(ph.base, __float_as_int(ph.f), o % (x ? nphx : nphy), 0);
+}
+
 // (kernels without anti-ringing: at least 6 waves per SIMD, a register budget their
 // out-of-line re-decision inherits.)  TAB: the plan has the pre-interpolated weight table;
 // separate instances keep the per-pixel bilinear fetch's registers out of the table path.
@@ -148,7 +162,7 @@ template <int R, bool AR, int CH, bool TAB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6))) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
                                                         const __half* __restrict__ lut_ar,
                                                         const float4* __restrict__ wtab, int nphx, int nphy,
-                                                        int out_w, int out_h) {
+                                                        const int4* __restrict__ phase, int out_w, int out_h) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
   constexpr int HALF = N * N / 2;
@@ -165,11 +179,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
   const BlockXYZ bk3 = block_xyz((out_w + ZT_X - 1) / ZT_X, (out_h + ZT_Y - 1) / ZT_Y, io.batch);
   const int f = bk3.z;
   const int ox0 = bk3.x * ZT_X, oy0 = bk3.y * ZT_Y;
-  const int bx0 = phase_of(ox0, io.W, out_w).base;
-  const int by0 = phase_of(oy0, io.H, out_h).base;
+  const int4* phy_tab = phase + out_w;
+  const int bx0 = phase[ox0].x;
+  const int by0 = phy_tab[oy0].x;
   // base pixels this tile's (in-range) outputs touch: nbx x nby <= ZB_X x ZB_Y (in < out)
-  const int nbx = phase_of(min(ox0 + ZT_X, out_w) - 1, io.W, out_w).base - bx0 + 1;
-  const int nby = phase_of(min(oy0 + ZT_Y, out_h) - 1, io.H, out_h).base - by0 + 1;
+  const int nbx = phase[min(ox0 + ZT_X, out_w) - 1].x - bx0 + 1;
+  const int nby = phy_tab[min(oy0 + ZT_Y, out_h) - 1].x - by0 + 1;
   const void* in = frame_in(io, f);
   {
     HaloTile<P, PH, PW, 256> h;
@@ -211,19 +226,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
 
   const int ox = ox0 + threadIdx.x % ZT_X, oy = oy0 + threadIdx.x / ZT_X;
   if (ox >= out_w || oy >= out_h) return;
-  const Phase phx = phase_of(ox, io.W, out_w), phy = phase_of(oy, io.H, out_h);
-  const int lbx = phx.base - bx0, lby = phy.base - by0;  // in [0, nbx) x [0, nby) by construction
+  const int4 phx = phase[ox], phy = phy_tab[oy];
+  const float fx = __int_as_float(phx.y), fy = __int_as_float(phy.y);
+  const int lbx = phx.x - bx0, lby = phy.x - by0;  // in [0, nbx) x [0, nby) by construction
   const int row = bucket[lby * ZB_X + lbx];
   if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(oy) * out_w + ox] = row;
   auto si = [&](int c, int i) { return c * PN + (lby + i % N) * PW + lbx + i / N; };
-  const float txf = 8.f * phx.f, txi = 8.f * (1.f - phx.f);
-  const float tyf = 8.f * phy.f, tyi = 8.f * (1.f - phy.f);
+  const float txf = 8.f * fx, txi = 8.f * (1.f - fx);
+  const float tyf = 8.f * fy, tyi = 8.f * (1.f - fy);
   // this pixel's entries of the pre-interpolated table (when the plan has one)
   constexpr int NT = AR ? 4 : 2;
   const float4* wt = nullptr;
   if constexpr (TAB) {
     const int trow = min(max(row, kZoomRowLo), kZoomRowLo + kZoomRows - 1) - kZoomRowLo;
-    wt = wtab + static_cast<int64_t>(((trow * nphy + oy % nphy) * nphx + ox % nphx) * GROUPS) * NT;
+    wt = wtab + static_cast<int64_t>(((trow * nphy + phy.z) * nphx + phx.z) * GROUPS) * NT;
   }
   float res[P], hi[P], lo[P], hi2[P], lo2[P];
 #pragma unroll
@@ -261,15 +277,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
         const int q = N * N - 1 - p;
 #pragma unroll
         for (int c = 0; c < P; ++c) {
+          // one fused multiply-add per tap and sum (the sums' rounding order is free: the
+          // results are held to the FP64 oracle within a tolerance)
           const float lp = tile[si(c, p)], lq = tile[si(c, q)];
-          res[c] += lp * wf[k] + lq * wi[k];
+          res[c] = fmaf(lq, wi[k], fmaf(lp, wf[k], res[c]));
           if (AR) {
             // ravu-zoom-ar-r2-rgb.hook:152-222: (0.1+l)^32, (1.1-l)^32 and the ^33 terms
             const float4 a = pw[si(c, p)], b = pw[si(c, q)];
-            hi[c] += a.x * af[k] + b.x * ai[k];
-            lo[c] += a.y * af[k] + b.y * ai[k];
-            hi2[c] += a.z * af[k] + b.z * ai[k];
-            lo2[c] += a.w * af[k] + b.w * ai[k];
+            hi[c] = fmaf(b.x, ai[k], fmaf(a.x, af[k], hi[c]));
+            lo[c] = fmaf(b.y, ai[k], fmaf(a.y, af[k], lo[c]));
+            hi2[c] = fmaf(b.z, ai[k], fmaf(a.z, af[k], hi2[c]));
+            lo2[c] = fmaf(b.w, ai[k], fmaf(a.w, af[k], lo2[c]));
           }
         }
       }
@@ -298,15 +316,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
 
 hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
                             const ZoomPlan& zp, int out_w, int out_h, hipStream_t st) {
+  if (!zp.phase) return hipErrorInvalidValue;
   dim3 grid(((out_w + ZT_X - 1) / ZT_X) * ((out_h + ZT_Y - 1) / ZT_Y) * io.batch);
 #define MPX_ZOOM(R, A, CH)                                                                             \
   if (r == R && ar == A && channels == CH) {                                                           \
     if (zp.wtab)                                                                                       \
       hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, true>), grid, dim3(256), 0, st, io, lut, lut_ar,    \
-                         reinterpret_cast<const float4*>(zp.wtab), zp.nphx, zp.nphy, out_w, out_h);      \
+                         reinterpret_cast<const float4*>(zp.wtab), zp.nphx, zp.nphy, zp.phase, out_w,    \
+                         out_h);                                                                         \
     else                                                                                               \
       hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, false>), grid, dim3(256), 0, st, io, lut, lut_ar,   \
-                         nullptr, 0, 0, out_w, out_h);                                                   \
+                         nullptr, 0, 0, zp.phase, out_w, out_h);                                         \
     return hipGetLastError();                                                                          \
   }
   MPX_ZOOM(2, false, 0) MPX_ZOOM(2, false, 1) MPX_ZOOM(2, false, 2) MPX_ZOOM(3, false, 0) MPX_ZOOM(3, false, 1)
@@ -320,6 +340,25 @@ int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w
   ZoomPlan& zp = plan->zoom;
   zp = ZoomPlan{};
   const int nphx = out_w / std::gcd(in_w, out_w), nphy = out_h / std::gcd(in_h, out_h);
+  {
+    const size_t pbytes = static_cast<size_t>(out_w + out_h) * sizeof(int4);
+    void* pp = nullptr;
+    if (int rc = ctx->alloc(pbytes, &pp)) return rc;
+    mpx_host::DeviceScope ds;
+    hipError_t e = hipErrorInvalidDevice;
+    if (!ds.enter(ctx->device)) {
+      hipLaunchKernelGGL(zoom_phase_kernel, dim3((out_w + out_h + 255) / 256), dim3(256), 0, nullptr,
+                         static_cast<int4*>(pp), in_w, in_h, out_w, out_h, nphx, nphy);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    }
+    if (e != hipSuccess) {
+      ctx->release(pp, pbytes);
+      return mpx_host::fail(MPX_ERR_HIP, std::string("ravu-zoom phase table: ") + hipGetErrorString(e));
+    }
+    zp.phase = static_cast<int4*>(pp);
+    zp.phase_bytes = pbytes;
+  }
   if (static_cast<int64_t>(nphx) * nphy > kZoomMaxPhasePairs) return MPX_OK;  // per-pixel interpolation
   const int r = plan->v->radius;
   const bool ar = plan->v->ar != 0;
@@ -332,6 +371,7 @@ int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w
   int rc;
   if ((rc = ds.enter(ctx->device))) {
     ctx->release(p, bytes);
+    zoom_plan_destroy(ctx, plan);
     return rc;
   }
   const dim3 grid((entries + 255) / 256);
@@ -352,6 +392,7 @@ int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) {
     ctx->release(p, bytes);
+    zoom_plan_destroy(ctx, plan);
     return mpx_host::fail(MPX_ERR_HIP, std::string("ravu-zoom weight table: ") + hipGetErrorString(e));
   }
   zp.wtab = static_cast<float*>(p);
@@ -363,6 +404,7 @@ int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w
 
 void zoom_plan_destroy(mpx_ctx* ctx, mpx_plan* plan) {
   ctx->release(plan->zoom.wtab, plan->zoom.wtab_bytes);
+  ctx->release(plan->zoom.phase, plan->zoom.phase_bytes);
   plan->zoom = ZoomPlan{};
 }
 

@@ -25,7 +25,7 @@ CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
            "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_plan_set_shape", "mpx_resample",
-           "mpx_last_error")
+           "mpx_weights_verify", "mpx_last_error")
 MAX_FRAMES = 16
 
 
@@ -84,6 +84,7 @@ def _load():
         "mpx_plan_set_shape": (i32, [vp, i32, i32]),
         "mpx_resample": (i32, [ctypes.POINTER(MpxIo), i32, i32, i32, i32, i32, i32, i32, i32, i32,
                                ctypes.c_float, ctypes.c_float, vp]),
+        "mpx_weights_verify": (i32, [ctypes.c_char_p, ctypes.c_char_p]),
         "mpx_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():

@@ -100,6 +100,22 @@ class Plan:
         (0 = occupancy maximum) and, for NNEDI3 nns 64 / 128, the neuron split."""
         check(LIB.mpx_plan_set_shape(self._h, int(grid_per_cu), int(split)))
 
+    def _check_frame(self, x):
+        """One frame of run_frames / run_frames_sized / the serving loop: a contiguous
+        device tensor of the plan's element type, (H, W) for a luma plan and (3, H, W)
+        for a -yuv / -rgb plan (the kernels read `planes` planes of pitch * height)."""
+        planes = self.info["planes"]
+        if not x.is_cuda:
+            raise ValueError("frame must be a device tensor")
+        if x.dtype != self.in_dtype:
+            raise TypeError("plan expects %s frames, got %s" % (self.in_dtype, x.dtype))
+        if planes == 1 and x.dim() != 2:
+            raise ValueError("a luma plan takes (H, W) frames, got shape %s" % (tuple(x.shape),))
+        if planes == 3 and (x.dim() != 3 or x.shape[0] != 3):
+            raise ValueError("a 3-plane plan takes (3, H, W) frames, got shape %s" % (tuple(x.shape),))
+        if not x.is_contiguous():
+            raise ValueError("frames must be contiguous")
+
     def _io(self, x, out, buckets):
         planes = self.info["planes"]
         if x.dtype != self.in_dtype:
@@ -147,13 +163,20 @@ class Plan:
         n = len(xs)
         if n < 1 or n > min(_abi.MAX_FRAMES, self.max_batch):
             raise ValueError("run_frames takes 1..%d frames" % min(_abi.MAX_FRAMES, self.max_batch))
+        for x in xs:
+            self._check_frame(x)
+            if tuple(x.shape[-2:]) != (self.in_h, self.in_w):
+                raise ValueError("frame of shape %s does not match the plan's %dx%d" % (tuple(x.shape), self.in_w,
+                                                                                        self.in_h))
         if outs is None:
             outs = [torch.empty(self.out_shape(x), dtype=self.out_dtype, device=x.device) for x in xs]
+        if len(outs) != n:
+            raise ValueError("need one output per frame")
         pitch_in = xs[0].stride(-2)
         pitch_out = outs[0].stride(-2)
         for x, o in zip(xs, outs):
-            if x.dtype != self.in_dtype or tuple(x.shape[-2:]) != (self.in_h, self.in_w) or not x.is_contiguous():
-                raise ValueError("frame does not match the plan")
+            if o.dtype != self.out_dtype or tuple(o.shape) != self.out_shape(x) or not o.is_cuda:
+                raise ValueError("bad output tensor")
             if x.stride(-2) != pitch_in or o.stride(-2) != pitch_out or not o.is_contiguous():
                 raise ValueError("frames must share row pitches")
         ins = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
@@ -176,11 +199,15 @@ class Plan:
         n = len(xs)
         if n < 1 or n > min(_abi.MAX_FRAMES, self.max_batch):
             raise ValueError("run_frames_sized takes 1..%d frames" % min(_abi.MAX_FRAMES, self.max_batch))
+        for x in xs:
+            self._check_frame(x)
         if outs is None:
             outs = [torch.empty(self.frame_out_shape(x), dtype=self.out_dtype, device=x.device) for x in xs]
+        if len(outs) != n:
+            raise ValueError("need one output per frame")
         for x, o in zip(xs, outs):
-            if x.dtype != self.in_dtype or not x.is_contiguous() or not o.is_contiguous() or o.dtype != self.out_dtype:
-                raise ValueError("frame does not match the plan")
+            if not o.is_cuda or not o.is_contiguous() or o.dtype != self.out_dtype:
+                raise ValueError("bad output tensor")
             if tuple(o.shape) != self.frame_out_shape(x):
                 raise ValueError("bad output shape")
         ins = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])

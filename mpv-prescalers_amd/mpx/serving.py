@@ -21,7 +21,7 @@ import time
 import torch
 
 from ._abi import MpxError
-from .ops import Context, default_target
+from .ops import Context, default_target, query
 
 _BUDGET = -5  # MPX_ERR_BUDGET
 
@@ -95,9 +95,20 @@ class Server:
 
     # ------------------------------------------------------------------ client side
     def submit(self, name, frame, target=None, precision="fp32", bits=10, out_dtype=None):
-        """Queue one frame ((H, W) luma or (3, H, W)); returns a Request (``wait()``)."""
+        """Queue one frame ((H, W) luma or (3, H, W)); returns a Request (``wait()``).
+        A frame whose shape does not fit the variant's planes, or an unknown variant,
+        raises here (ValueError / MpxError) and nothing is queued."""
+        if frame.dim() < 2:
+            raise ValueError("frame must be (H, W) or (3, H, W)")
         in_h, in_w = frame.shape[-2:]
         tgt = tuple(target) if target is not None else default_target(name, in_w, in_h)
+        planes = query(name, in_w, in_h, tgt[0], tgt[1])["planes"]
+        if planes == 1 and frame.dim() != 2:
+            raise ValueError("%s takes (H, W) luma frames, got shape %s" % (name, tuple(frame.shape)))
+        if planes == 3 and (frame.dim() != 3 or frame.shape[0] != 3):
+            raise ValueError("%s takes (3, H, W) frames, got shape %s" % (name, tuple(frame.shape)))
+        if not frame.is_cuda or not frame.is_contiguous():
+            raise ValueError("frame must be a contiguous device tensor")
         if out_dtype is None:
             out_dtype = frame.dtype if frame.dtype in (torch.float16, torch.float32) else torch.float32
         r = Request(name, frame, tgt, precision, bits, out_dtype)

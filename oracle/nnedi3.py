@@ -2,6 +2,9 @@
 
 Test infrastructure only (see oracle/__init__.py).
 """
+import concurrent.futures
+import os
+
 import numpy as np
 
 from .common import load_weights, tex_off
@@ -24,16 +27,19 @@ def windows_y(plane, wh):
                     axis=-1)
 
 
-def predict(win, w1, w2, b1, b2, chunk=1 << 16):
+def predict(win, w1, w2, b1, b2, chunk=1 << 11):
     """nnedi3() of nnedi3-nns16-win8x4.hook:20-36:
     m = mean, v = E[x^2]-m^2, is = v >= eps ? v^-1/2 : 0, s = v*is;
     e_n = exp(dot(w1_n,x)*is + b1_n); t_n = dot(w2_n,x)*is + b2_n;
-    out = clamp(m + 5 * (sum e_n t_n/(1+|t_n|)) / (sum e_n) * s, 0, 1)."""
+    out = clamp(m + 5 * (sum e_n t_n/(1+|t_n|)) / (sum e_n) * s, 0, 1).
+    Pixels are independent: chunks of them run on a thread pool (NumPy releases the
+    interpreter lock inside its array operations), each chunk computed the same way."""
     shp = win.shape[:-1]
     K = win.shape[-1]
     x = win.reshape(-1, K)
     out = np.empty(x.shape[0])
-    for s in range(0, x.shape[0], chunk):
+
+    def run(s):
         xs = x[s:s + chunk]
         m = xs.sum(axis=1) / K
         v = (xs * xs).sum(axis=1) / K - m * m
@@ -46,7 +52,20 @@ def predict(win, w1, w2, b1, b2, chunk=1 << 16):
         wsum = e.sum(axis=1)
         vsum = (e * (s2 / (1.0 + np.abs(s2)))).sum(axis=1)
         out[s:s + chunk] = np.clip(m + 5.0 * vsum / wsum * sd, 0.0, 1.0)
+
+    starts = range(0, x.shape[0], chunk)
+    if x.shape[0] <= 2 * chunk:
+        for s in starts:
+            run(s)
+    else:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(1), concurrent.futures.ThreadPoolExecutor(_threads()) as ex:   # one BLAS thread per chunk
+            list(ex.map(run, starts))
     return out.reshape(shp)
+
+
+def _threads():
+    return max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)))
 
 
 def double_y(plane, nns, wh):

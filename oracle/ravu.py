@@ -240,7 +240,7 @@ def _bilinear_lut(lut, group, tx, ty):
     return (v00 * (1 - fx) + v01 * fx) * (1 - fy) + (v10 * (1 - fx) + v11 * fx) * fy
 
 
-def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None):
+def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None, cols=None, rows=None):
     """RAVU-zoom(-ar) to an arbitrary size (ravu-zoom-r2.hook:22-134,
     ravu-zoom-ar-r2-rgb.hook:25-224).
 
@@ -250,7 +250,10 @@ def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None):
     taps p < n^2/2 use phase f, tap n^2-1-p uses phase 1-f with the same (group, comp).
     Non-AR clamps to [0,1]; AR uses the separate learned `_ar` LUT (all taps, weights as
     stored), per channel, strength 0.8, no final clamp.  Base and phase follow from the
-    exact rational pos - 1/2 = ((2o+1)*in - out) / (2*out)."""
+    exact rational pos - 1/2 = ((2o+1)*in - out) / (2*out).
+    ``cols`` / ``rows`` (optional index arrays) evaluate only those output columns and
+    rows (every output pixel depends on its own position alone): the result is the
+    (rows, cols) sub-grid of the full output."""
     planes = _as_planes(img, channels)
     C, H, Wd = planes.shape
     n = 2 * r
@@ -258,8 +261,8 @@ def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None):
     if ar:
         if ar_lut is None:
             ar_lut = load_lut16("ravu_zoom_lut%d_ar" % r)
-    ox = np.arange(out_w)
-    oy = np.arange(out_h)
+    ox = np.arange(out_w) if cols is None else np.asarray(cols, dtype=np.int64)
+    oy = np.arange(out_h) if rows is None else np.asarray(rows, dtype=np.int64)
     numx = (2 * ox + 1) * Wd - out_w
     numy = (2 * oy + 1) * H - out_h
     bx = numx // (2 * out_w)                            # base texel index (floor)
@@ -282,8 +285,8 @@ def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None):
     # unclamped bucket (angle can be -1 or 24 at the mod wrap): the LINEAR fetch clamps
     # texel rows to the texture, not the bucket index
     bucket = ((angle * 4 + strength) * 3 + coherence).astype(np.int64)   # (out_h, out_w)
-    FX = np.broadcast_to(fx[None, :], (out_h, out_w))
-    FY = np.broadcast_to(fy[:, None], (out_h, out_w))
+    FX = np.broadcast_to(fx[None, :], (len(oy), len(ox)))
+    FY = np.broadcast_to(fy[:, None], (len(oy), len(ox)))
     ty = 9 * bucket + 8 * FY
     ty_inv = 9 * bucket + 8 * (1 - FY)
     res = 0.0

@@ -47,3 +47,18 @@ def compare(case, got, ref, got_rows=None, ref_rows=None, site_shape=None, site_
             fields["max_abs_agreeing"] = float(err[mask].max()) if mask.any() else 0.0
     report(case, **fields)
     return fields
+
+
+def nnedi3_crop_ref(img, nns, wh, ox0, oy0, ow, oh, margin=8):
+    """FP64 oracle output of the 2x NNEDI3 hook over the output region
+    [ox0, ox0+ow) x [oy0, oy0+oh) of a 2W x 2H result, computed on an input crop with
+    `margin` extra input pixels on every side that is not a frame edge.  A final pixel
+    depends on input pixels within 5 rows and 7 columns of its position (double_y window
+    rows -2..3 under double_x's rows -3..4; columns -3..4 under the x pass's -2..3), so
+    8 suffices and the region equals the full-frame oracle's."""
+    from oracle.nnedi3 import nnedi3
+    H, W = img.shape
+    x0, y0 = max(ox0 // 2 - margin, 0), max(oy0 // 2 - margin, 0)
+    x1, y1 = min((ox0 + ow + 1) // 2 + margin, W), min((oy0 + oh + 1) // 2 + margin, H)
+    out = nnedi3(np.asarray(img[y0:y1, x0:x1], np.float64), nns, wh)
+    return out[oy0 - 2 * y0:oy0 - 2 * y0 + oh, ox0 - 2 * x0:ox0 - 2 * x0 + ow]

@@ -43,3 +43,30 @@ def test_sized_launch_rejections():
     lite = ctx.plan("ravu-lite-r2", 64, 48, max_batch=2)
     with pytest.raises(MpxError):
         lite.run_frames_sized([torch.rand((49, 64), device="cuda")])   # taller than the plan
+
+
+@pytest.mark.parametrize("name,planes", [("ravu-r3-yuv", 3), ("ravu-lite-r3", 1), ("nnedi3-nns32-win8x4", 1)])
+def test_frame_planes_checked_before_launch(name, planes):
+    """run_frames / run_frames_sized / Server.submit reject a frame whose planes do not
+    match the plan (a luma frame sent to a 3-plane plan would make the kernels read and
+    write two planes past the tensors): ValueError, and nothing is launched."""
+    import mpx
+    from mpx.serving import Server
+    ctx = mpx.Context(0)
+    plan = ctx.plan(name, 64, 48, in_dtype=torch.float16, out_dtype=torch.float16, max_batch=4)
+    wrong = [torch.rand((48, 64), device="cuda", dtype=torch.float16)] if planes == 3 else \
+        [torch.rand((3, 48, 64), device="cuda", dtype=torch.float16), torch.rand((2, 48, 64), device="cuda",
+                                                                                  dtype=torch.float16)]
+    for x in wrong:
+        with pytest.raises(ValueError):
+            plan.run_frames([x])
+        with pytest.raises(ValueError):
+            plan.run_frames_sized([x])
+    srv = Server(device=0, hbm_budget=1 << 30, streams=1, max_batch=2)
+    try:
+        for x in wrong:
+            with pytest.raises(ValueError):
+                srv.submit(name, x)
+        assert srv.launches == 0
+    finally:
+        srv.close()
