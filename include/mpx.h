@@ -199,6 +199,21 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
                          const int* heights, const int64_t* in_pitches, const int64_t* out_pitches,
                          void* hip_stream);
 
+/* mpx_run_ex with the plan's intermediates (the NNEDI3 double_y plane when both axes
+ * double; the RAVU int11 plane) in a device array that the library holds in its code
+ * object, MPX_STATIC_SCRATCH_BYTES per device, instead of the plan's scratch.  It calls
+ * no allocating routine of HIP or of any library.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  MPX_ERR_SHAPE, before anything is
+ * enqueued, when the batch's intermediates exceed MPX_STATIC_SCRATCH_BYTES (e.g. 4
+ * frames of 1920x1080 NNEDI3 2x need 4 x 16.6 MB; one 2560x1440 ravu-rN-yuv frame needs
+ * 44.2 MB).  Every call on one device shares that device's array: calls whose work
+ * overlaps on one device (two streams, or two host threads) race on it and give wrong
+ * results; serialise them.  Nothing in the array needs to start at zero (each pass writes
+ * every element it is read back from).  Plans without intermediates run as mpx_run_ex. */
+#define MPX_STATIC_SCRATCH_BYTES (64u << 20)
+int mpx_run_static(mpx_plan* plan, const mpx_io* io, void* hip_stream);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* mpx_last_error(void);
 

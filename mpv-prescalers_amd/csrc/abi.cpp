@@ -141,6 +141,36 @@ constexpr int64_t kMaxLaunchPixels = (int64_t{1} << 31) - 1;
 
 }  // namespace
 
+// ----------------------------------------------------------------------------- static scratch
+
+// The intermediates of mpx_run_static.  The array lives in this file's code object, so each
+// device that loads the library holds its own copy until the process ends, and no call
+// allocates it.  On the host, `scratch` names a host-side stand-in: kernels are given the
+// device address that hipGetSymbolAddress returns for the current device, never the name.
+static __device__ __align__(256) unsigned char scratch[MPX_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device -> device address of its `scratch`
+
+// The device address of `device`'s copy of `scratch`, resolved once per device with that
+// device current (the caller has made it current).
+int scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(MPX_ERR_HIP, "hipGetSymbolAddress returned a NULL address for the static scratch");
+    it = g_scratch_ptr.emplace(device, p).first;
+  }
+  *out = it->second;
+  return MPX_OK;
+}
+
+}  // namespace
+
 // ----------------------------------------------------------------------------- context
 
 extern "C" {
@@ -185,8 +215,15 @@ int mpx_create(int device, uint64_t hbm_budget, const char* weights_dir, mpx_ctx
   int n = 0;
   HIP_TRY(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(MPX_ERR_ARG, "device " + std::to_string(device) + " out of range");
+  void* static_scratch = nullptr;
+  {
+    mpx_host::DeviceScope dscope;
+    if (int drc = dscope.enter(device)) return drc;
+    if (int src = scratch_address(device, &static_scratch)) return src;
+  }
   auto* c = new mpx_ctx();
   c->device = device;
+  c->static_scratch = static_scratch;
   c->budget = hbm_budget;
   c->wdir = weights_dir ? weights_dir : default_weights_dir();
   *out = c;
@@ -312,15 +349,28 @@ void mpx_plan_destroy(mpx_plan* plan) {
   delete plan;
 }
 
-static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* scratch);
+static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* interm);
 
 int mpx_run_ex(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
   return mpx_host::run_io(plan, io, hip_stream, plan ? plan->scratch : nullptr);
 }
 
+int mpx_run_static(mpx_plan* plan, const mpx_io* io, void* hip_stream) {
+  if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
+  if (io->batch >= 1 && plan->scratch_frame) {
+    const int64_t need = plan->scratch_frame * 4 * static_cast<int64_t>(io->batch);
+    if (need > static_cast<int64_t>(MPX_STATIC_SCRATCH_BYTES))
+      return fail(MPX_ERR_SHAPE, "intermediates of " + std::to_string(io->batch) + " frame(s) need " +
+                                     std::to_string(need) + " B, more than the static scratch's " +
+                                     std::to_string(MPX_STATIC_SCRATCH_BYTES) + " B");
+  }
+  if (!plan->ctx->static_scratch) return fail(MPX_ERR_HIP, "the context has no static scratch address");
+  return mpx_host::run_io(plan, io, hip_stream, plan->ctx->static_scratch);
+}
+
 }  // extern "C"
 
-int mpx_host::run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* scratch) {
+int mpx_host::run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* interm) {
   if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
   if (!io->in || !io->out) return fail(MPX_ERR_ARG, "in/out is NULL");
   if (io->batch < 1 || io->batch > plan->max_batch)
@@ -358,7 +408,7 @@ int mpx_host::run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* s
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = io->batch;
-  return dispatch(plan, d, st, scratch);
+  return dispatch(plan, d, st, interm);
 }
 
 extern "C" {
@@ -444,7 +494,7 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
   return dispatch(plan, d, static_cast<hipStream_t>(hip_stream), plan->scratch);
 }
 
-static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* scratch) {
+static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* interm) {
   const mpx_variant_info& inf = plan->info;
   hipError_t e = hipSuccess;
   if (!inf.applies) {
@@ -459,7 +509,7 @@ static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* scra
         e = mpx::launch_ravu_3x(v.radius, v.channels, d, plan->lut, st);
         break;
       case MPX_FAMILY_RAVU:
-        e = mpx::launch_ravu(v.radius, v.channels, d, plan->lut, static_cast<float*>(scratch), plan->scratch_frame,
+        e = mpx::launch_ravu(v.radius, v.channels, d, plan->lut, static_cast<float*>(interm), plan->scratch_frame,
                              st);
         break;
       case MPX_FAMILY_RAVU_ZOOM:
@@ -467,7 +517,7 @@ static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* scra
                                   inf.out_w, inf.out_h, st);
         break;
       case MPX_FAMILY_NNEDI3:
-        e = mpx::launch_nnedi3(plan->nw, d, inf.scale_x == 2, inf.scale_y == 2, static_cast<float*>(scratch),
+        e = mpx::launch_nnedi3(plan->nw, d, inf.scale_x == 2, inf.scale_y == 2, static_cast<float*>(interm),
                                plan->scratch_frame, st);
         break;
     }

@@ -77,9 +77,9 @@ std::string sha256_of(const void* data, size_t len);
 
 struct mpx_plan;
 namespace mpx_host {
-// mpx_run_ex with the intermediates in `scratch` (plan->scratch_frame FP32 elements per
+// mpx_run_ex with the intermediates in `interm` (plan->scratch_frame FP32 elements per
 // frame; the caller has checked its size).
-int run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* scratch);
+int run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* interm);
 }  // namespace mpx_host
 
 using mpx_host::fail;
@@ -94,6 +94,7 @@ struct mpx_ctx {
   uint64_t budget = 0;
   uint64_t used = 0;
   std::string wdir;
+  void* static_scratch = nullptr;  // device address of this device's array for mpx_run_static
   std::mutex mu;    // weight tables
   std::mutex amu;   // budget accounting (taken inside mu, never the other way round)
   std::map<std::string, DeviceBuf> weights;            // resident weight tables

@@ -25,8 +25,9 @@ CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
            "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_plan_set_shape", "mpx_resample",
-           "mpx_weights_verify", "mpx_last_error")
+           "mpx_weights_verify", "mpx_run_static", "mpx_last_error")
 MAX_FRAMES = 16
+STATIC_SCRATCH_BYTES = 64 << 20   # MPX_STATIC_SCRATCH_BYTES
 
 
 class VariantInfo(ctypes.Structure):
@@ -76,6 +77,7 @@ def _load():
         "mpx_plan_destroy": (None, [vp]),
         "mpx_run": (i32, [vp, vp, i64, i64, vp, i64, i64, i32, vp]),
         "mpx_run_ex": (i32, [vp, ctypes.POINTER(MpxIo), vp]),
+        "mpx_run_static": (i32, [vp, ctypes.POINTER(MpxIo), vp]),
         "mpx_run_frames": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), i64, i64, vp]),
         "mpx_run_frames_sized": (i32, [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i32),
                                        ctypes.POINTER(i32), ctypes.POINTER(i64), ctypes.POINTER(i64), vp]),

@@ -83,6 +83,17 @@ class Plan:
         check(LIB.mpx_run_ex(self._h, ctypes.byref(io), ctypes.c_void_p(stream.cuda_stream)))
         return (out, bk) if buckets else out
 
+    def run_static(self, x, out=None, stream=None):
+        """``run`` through mpx_run_static: the intermediates go to the library's static
+        per-device array (at most 64 MiB per launch) instead of the plan's scratch, and the
+        call allocates nothing but ``out`` when it is None.  Calls that overlap on one
+        device race on that array."""
+        io, x, out, _ = self._io(x, out, False)
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        check(LIB.mpx_run_static(self._h, ctypes.byref(io), ctypes.c_void_p(stream.cuda_stream)))
+        return out
+
     def tune(self, x, out=None, iters=5, stream=None):
         """Autotune the launch shape on ``x`` / ``out`` (writes ``out``); the plan keeps the
         fastest.  Returns {"grid_per_cu", "split", "us"} (grid 0 = occupancy maximum;
