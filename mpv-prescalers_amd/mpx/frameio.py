@@ -16,10 +16,12 @@ _CS = {  # y4m colourspace tag -> (chroma subsampling x, y, bits)
 
 
 def _parse_cs(tag):
+    """(sx, sy, bits) of a colour-space tag: 420/422/444 with optional pNN (ffmpeg's
+    C420p10, C444p16 ...), mono and monoNN (ffmpeg's Cmono16)."""
     if tag in _CS:
         return _CS[tag]
-    m = re.match(r"^(420|422|444|mono)p(\d+)$", tag)
-    if not m:
+    m = re.match(r"^(420|422|444)p(\d+)$", tag) or re.match(r"^(mono)p?(\d+)$", tag)
+    if not m or not 8 <= int(m.group(2)) <= 16:
         raise ValueError("unsupported y4m colourspace C%s" % tag)
     sx, sy, _ = _CS[m.group(1)]
     return sx, sy, int(m.group(2))
