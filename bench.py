@@ -37,6 +37,39 @@ RAVU_SECONDARY = (("ravu-lite-ar-r3", (640, 360), None, 1, 64, "float32"),
                   ("ravu-zoom-r3-rgb", (1920, 1080), (2880, 1620), 3, 1, "float16"))
 
 
+def parity_record(case_prefix):
+    """max-abs / PSNR of a benchmark workload at full size against the FP64 oracle, as the
+    GPU tests recorded them (profiles/parity_full_size.json), or None."""
+    path = os.path.join(ROOT, "profiles", "parity_full_size.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for r in json.load(f)["records"]:
+            if r["case"].startswith(case_prefix):
+                out = {k: r[k] for k in ("max_abs", "psnr", "tol", "bucket_agree") if k in r}
+                out["case"] = r["case"]
+                return out
+    return None
+
+
+def zoom_ar3_weights_dir(tmp):
+    """A weights directory holding the shipped tables and a supplied ravu-zoom r3
+    anti-ringing table (the reference does not ship one): |main r3 table|, the synthetic
+    table the GPU tests check against the oracle."""
+    import importlib.util
+    import numpy as np
+    wdir = os.path.join(ROOT, "mpv-prescalers_amd", "weights")
+    spec = importlib.util.spec_from_file_location("mpx_extract", os.path.join(wdir, "extract.py"))
+    ext = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ext)
+    for f in os.listdir(wdir):
+        if f.endswith(".bin"):
+            os.symlink(os.path.join(wdir, f), os.path.join(tmp, f))
+    ext.write_bin(os.path.join(tmp, "ravu_zoom_lut3_ar.bin"),
+                  np.abs(ext.read_bin(os.path.join(wdir, "ravu_zoom_lut3.bin"))).astype(np.float32))
+    return tmp
+
+
 def nnedi3_flop_per_px(name):
     nns = int(name.split("nns")[1].split("-")[0])
     k = 8 * int(name.split("win8x")[1])
@@ -276,11 +309,13 @@ def dominant_kernel(ctx, steps, torch):
     achieved = flop / (kms * 1e-3) / 1e12
     traffic = None
     valu_share = None
+    mfma_busy = None
     tf = os.path.join(ROOT, "profiles", "traffic_nnedi3_double_x_nns32_1920x2160.json")
     if os.path.exists(tf):
         with open(tf) as f:
             rec = json.load(f)
         traffic = rec.get("hbm_bytes_per_launch")
+        mfma_busy = rec.get("mfma_busy_frac")
         if rec.get("valu_instructions_per_launch"):
             # issue model: 4 cycles per VALU instruction, 16 per transcendental, over the
             # 1024 SIMDs at 2.4 GHz, as a share of the live kernel time
@@ -291,7 +326,7 @@ def dominant_kernel(ctx, steps, torch):
             "kernel": "nnedi3_mfma double_x (fp16, nns32, win8x4)", "kernel_ms": round(kms, 5),
             "flop_per_launch": flop, "algorithmic_io_bytes": W * 2 * H * 4 + 2 * W * 2 * H * 2,
             "binding_resource": "VALU issue (FP32 normalisation + exp2/rcp epilogue)",
-            "valu_issue_share": valu_share}
+            "valu_issue_share": valu_share, "mfma_busy_frac": mfma_busy}
 
 
 def kernel_only(ctx, steps, torch):
@@ -405,16 +440,23 @@ def main():
         f10 = torch.from_numpy(np.stack([make_plane(W, H, seed=20 + i, bits=10) for i in range(b)])
                                .astype(np.float16)).to(dev)
         o10 = torch.empty((b, 2 * H, 2 * W), dtype=torch.float16, device=dev)
-        for p in ("fp16", "bf16", "fp8", "fp32"):
+        accuracy_forms = {}
+        for p in ("fp16", "bf16", "fp16x3", "fp8", "fp32"):
             pl = ctx.plan("nnedi3-nns256-win8x6", W, H, in_dtype=torch.float16, out_dtype=torch.float16,
                           precision=p, max_batch=b)
             n = 2 if p == "fp32" else 5
             pl.tune(f10, o10, iters=2)
             t, wall = timed(lambda: pl.run(f10, out=o10), n, 1, torch)
             fps = b * n / wall
-            secondary["nnedi3-nns256-win8x6 %s b8" % p] = {
-                "fps": round(fps, 1), "mpx_s": round(fps * 4 * W * H / 1e6, 1),
-                "tflops": round(fps * nnedi3_flop_per_px("nnedi3-nns256-win8x6") * 3 * W * H / 1e12, 1)}
+            rec = {"fps": round(fps, 1), "mpx_s": round(fps * 4 * W * H / 1e6, 1),
+                   "tflops": round(fps * nnedi3_flop_per_px("nnedi3-nns256-win8x6") * 3 * W * H / 1e12, 1)}
+            err = parity_record("nnedi3-nns256-win8x6 %s 1920x1080 10-bit b8" % p)
+            if err:
+                rec["error_vs_fp64_oracle"] = err
+            # FP8 (e4m3 weights and window) is slower than FP16 on this kernel and ~40 dB
+            # less accurate: reported for its accuracy only
+            (accuracy_forms if p == "fp8" else secondary)["nnedi3-nns256-win8x6 %s b8" % p] = rec
+        secondary["accuracy_forms"] = accuracy_forms
         for name, (iw, ih), tgt, planes, b, dt in RAVU_SECONDARY:
             dt = getattr(torch, dt)
             fr = np.stack([make_plane(iw, ih, seed=i) for i in range(b * planes)]).astype(np.float32)
@@ -429,6 +471,28 @@ def main():
                 "mpx_s": round(b * 10 * opx / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
                 "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
                 "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4)}
+
+        # ravu-zoom-ar-r3-rgb at 2x and 1.5x on a supplied (synthetic) anti-ringing table
+        import tempfile
+        with tempfile.TemporaryDirectory() as tmp:
+            zctx = mpx.Context(0, weights_dir=zoom_ar3_weights_dir(tmp))
+            for (iw, ih), (tw, th), b in (((1280, 720), (2560, 1440), 4), ((1920, 1080), (2880, 1620), 2)):
+                fr = np.stack([make_plane(iw, ih, seed=90 + i, bits=8) for i in range(3 * b)]).astype(np.float16)
+                xin = torch.from_numpy(fr.reshape(b, 3, ih, iw)).to(dev)
+                pl = zctx.plan("ravu-zoom-ar-r3-rgb", iw, ih, target=(tw, th), in_dtype=torch.float16,
+                               out_dtype=torch.float16, max_batch=b)
+                t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
+                io_bytes = (iw * ih + tw * th) * 3 * 2
+                rec = {"mpx_s": round(b * 10 * tw * th / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
+                       "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
+                       "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4),
+                       "weights": "synthetic AR table (|r3 main table|; the reference ships none)"}
+                err = parity_record("ravu-zoom-ar-r3-rgb %dx%d->%dx%d" % (iw, ih, tw, th))
+                if err:
+                    rec["error_vs_fp64_oracle"] = err
+                secondary["ravu-zoom-ar-r3-rgb %dx%d->%dx%d b%d" % (iw, ih, tw, th, b)] = rec
+                pl.close()
+            zctx.close()
 
     serving = None
     if not args.no_serving:

@@ -26,9 +26,10 @@ struct NnediWeights {
   const float* bias = nullptr;
   const int* wexp = nullptr;
   // launch shape (set by the autotuner; 0 = defaults): persistent blocks per CU (capped at
-  // the kernel's occupancy), and for nns = 64 / 128 the neuron split (nns 64 -- 0: every
-  // wave holds all 4 tiles, 1: 2 slices x 2 tiles; nns 128 -- 0: 2 slices x 4 tiles, 1: 4
-  // slices x 2 tiles per wave)
+  // the kernel's occupancy), and for nns = 64 / 128 the neuron split over the block's 4
+  // waves (0: fewer slices, more neuron tiles per wave; 1: more slices).  16x16 tiles --
+  // nns 64: 1 x 4 or 2 x 2 (slices x tiles), nns 128: 2 x 4 or 4 x 2; 32x32 tiles -- nns 64:
+  // 1 x 2 or 2 x 1, nns 128: 2 x 2 or 4 x 1
   int grid_per_cu = 0;
   int split = 0;
 };

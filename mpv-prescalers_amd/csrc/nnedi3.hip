@@ -15,8 +15,9 @@
 //
 // Two forms:
 //  * nnedi3_valu: plain FP32, one thread per pixel, weights read as wave-uniform scalars.
-//  * nnedi3_mfma: the [neurons x K] x [K x pixels] contraction on MFMA
-//    (v_mfma_f32_16x16x32_{f16,bf16}; K = 48 as two 32-chunks, zero padded), FP32
+//  * nnedi3_mfma / nnedi3_mfma32: the [neurons x K] x [K x pixels] contraction on MFMA
+//    (v_mfma_f32_16x16x32_{f16,bf16}, K = 48 as two 32-chunks, zero padded; or
+//    v_mfma_f32_32x32x16_{f16,bf16}, K = 48 as three 16-chunks), FP32
 //    accumulation, with the biases as the accumulator's initial value; weights and inputs
 //    optionally split into high+low 16-bit parts (three products) so the contraction
 //    keeps ~22 (FP16) / ~16 (BF16) bits.  The exp2 /
@@ -589,6 +590,242 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   }
 }
 
+// ----------------------------------------------------------------------------- MFMA, 32x32 tiles
+
+typedef float float16v __attribute__((ext_vector_type(16)));
+
+// The contraction on v_mfma_f32_32x32x16_{f16,bf16}: K in chunks of 16, so the win8x6
+// window (K = 48) takes exactly three chunks where the 16x16x32 form pads it to 64 (a
+// quarter of its matrix work is zeros).  Lane l holds A row (neuron) l%32 and B column
+// (pixel) l%32 at k = 16*c + 8*(l/32) + 0..7, the same slots on both sides.  D: lane l
+// holds column (pixel) l%32, rows 8*(r/4) + 4*(l/32) + r%4 for r = 0..15: 16 neurons of
+// one pixel.  No spare K slots: the biases are the accumulators' initial value (C).
+template <int FORM, int K, int TERMS>
+struct Mma32 {
+  using O = Op<FORM>;
+  using T = typename O::T;
+  using V8 = typename O::V8;
+  static_assert(K % 16 == 0, "K must be a multiple of 16");
+  static constexpr int KC = K / 16;
+  static constexpr int XS = K + 8;  // LDS row stride (elements): 16-byte aligned rows, spread banks
+  static constexpr int LO_ROWS = TERMS >= 2;
+  static constexpr bool FUSED_SCALE = FORM == F_F16 && TERMS <= 2;
+  struct W {
+    V8 hi[KC], lo[KC];
+  };
+  struct B {
+    V8 h[KC], l[KC];
+  };
+  __device__ static float16v mma(V8 a, V8 b, float16v c) {
+    if constexpr (FORM == F_F16) return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  __device__ static void load_w(W& w, const void* wh, const void* wl, int n, int mtx, int half) {
+    const T* H = static_cast<const T*>(wh);
+    const T* L = static_cast<const T*>(wl);
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int64_t i = (static_cast<int64_t>(n) * 2 + mtx) * K + 16 * c + 8 * half + j;
+        w.hi[c][j] = H[i];
+        w.lo[c][j] = TERMS >= 3 ? L[i] : static_cast<T>(0.0f);
+      }
+  }
+  // one pixel's window -> its LDS row(s): x' * xscale (SCALED) or x', split into 16-bit
+  // high and low parts; with FUSED_SCALE the window is centred only and 1/std = is is
+  // applied inside the conversions
+  __device__ static void write_row(T* rh, T* rl, const float* xv, float xscale, bool scaled, float is) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if constexpr (FUSED_SCALE) {
+        const T h = O::cvt(fmaf(xv[k], is, 0.0f));
+        rh[k] = h;
+        if (TERMS >= 2) rl[k] = O::cvt(fmaf(xv[k], is, -O::back(h)));
+      } else {
+        const float v = scaled ? xv[k] * xscale : xv[k];
+        const T h = O::cvt(v);
+        rh[k] = h;
+        if (TERMS >= 2) rl[k] = O::cvt(v - O::back(h));
+      }
+    }
+  }
+  __device__ static void load_b(B& b, const T* xh, const T* xl, int px, int half) {
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      b.h[c] = *reinterpret_cast<const V8*>(xh + px * XS + 16 * c + 8 * half);
+      if (TERMS >= 2) b.l[c] = *reinterpret_cast<const V8*>(xl + px * XS + 16 * c + 8 * half);
+    }
+  }
+  __device__ static float16v contract(const W& w, const B& b, float16v acc) {
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      acc = mma(w.hi[c], b.h[c], acc);
+      if (TERMS >= 2) acc = mma(w.hi[c], b.l[c], acc);
+      if (TERMS >= 3) acc = mma(w.lo[c], b.h[c], acc);
+    }
+    return acc;
+  }
+};
+
+// nnedi3_mfma on 32x32 tiles (F16 / BF16 forms).  Same block, staged tile, persistent
+// loop and double-buffered halo as nnedi3_mfma; the 256 pixels of a sub-tile form 8
+// groups of 32, and the nns/32 neuron tiles are split into NSL slices of TPW tiles per
+// wave (weights and biases in VGPRs for the launch).  NSL = 1: wave w owns groups 2w and
+// 2w+1, its own pixel row, so its LDS rows and sums stay within the wave; NSL > 1: waves
+// of one slice share the groups, partial sums meet in LDS.
+template <int WH, int AXIS, int FORM, int TERMS, int NSL, int TPW, bool SCALED>
+__global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __restrict__ w_hi,
+                                                        const void* __restrict__ w_lo, const float* __restrict__ bias,
+                                                        float xscale, float inv_acc, int total_tiles) {
+  using G = Geo<WH, AXIS, NYS>;
+  constexpr int K = G::K;
+  using M = Mma32<FORM, K, TERMS>;
+  using T = typename M::T;
+  constexpr int XS = M::XS;
+  constexpr int NPX = NX * NY;      // 256 pixels per sub-tile
+  constexpr int NGROUP = NPX / 32;  // 8 pixel groups
+  constexpr int PSL = 4 / NSL;      // pixel slices
+  static_assert(NSL * PSL == 4, "4 waves per block");
+  __shared__ float tile[2][G::TH * G::TW];
+  __shared__ __attribute__((aligned(16))) T xhi[NPX * XS];
+  __shared__ __attribute__((aligned(16))) T xlo[M::LO_ROWS ? NPX * XS : 8];
+  __shared__ float part[NSL > 1 ? NSL : 1][NSL > 1 ? NPX : 1][2];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nslice = wave % NSL;
+  const int pslice = wave / NSL;
+  const int col = lane & 31, half = lane >> 5;
+
+  typename M::W w1[TPW], w2[TPW];
+  float16v b1[TPW], b2[TPW];
+#pragma unroll
+  for (int t = 0; t < TPW; ++t) {
+    const int n0 = (nslice * TPW + t) * 32;
+    M::load_w(w1[t], w_hi, w_lo, n0 + col, 0, half);
+    M::load_w(w2[t], w_hi, w_lo, n0 + col, 1, half);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int nn = n0 + 8 * (r / 4) + 4 * half + (r % 4);  // D row of this lane's value r
+      b1[t][r] = bias[2 * nn];
+      b2[t][r] = bias[2 * nn + 1];
+    }
+  }
+  // Four neurons' softmax weights (added to ws) and elliott terms as one fraction n / d
+  // (bounds as in nnedi3_mfma: d <= 64^4, n <= 4 * 2^26 * 64^4 per quad).
+  auto quad_frac = [&](const float16v& a1, const float16v& a2, int r0, float& ws, float& n, float& d) {
+    float e[4], q[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      e[i] = __builtin_amdgcn_exp2f(SCALED ? a1[r0 + i] * inv_acc : a1[r0 + i]);
+      ws += e[i];
+      q[i] = SCALED ? fmaf(fabsf(a2[r0 + i]), inv_acc, 1.0f) : 1.0f + fabsf(a2[r0 + i]);
+    }
+    const float n01 = fmaf(e[0] * a2[r0], q[1], e[1] * a2[r0 + 1] * q[0]), d01 = q[0] * q[1];
+    const float n23 = fmaf(e[2] * a2[r0 + 2], q[3], e[3] * a2[r0 + 3] * q[2]), d23 = q[2] * q[3];
+    n = fmaf(n01, d23, n23 * d01);
+    d = d01 * d23;
+  };
+  // softmax-weighted sums of pixel group g over this wave's neuron tiles; on return lanes
+  // l and l^32 hold the sums of pixel g*32 + l%32
+  auto group_sums = [&](int g, float& ws, float& vs) {
+    typename M::B bx;
+    M::load_b(bx, xhi, xlo, g * 32 + col, half);
+    ws = 0.f;
+    vs = 0.f;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const float16v a1 = M::contract(w1[t], bx, b1[t]);
+      const float16v a2 = M::contract(w2[t], bx, b2[t]);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {  // one reciprocal per 8 neurons
+        float n0, d0, n1, d1;
+        quad_frac(a1, a2, 8 * h, ws, n0, d0);
+        quad_frac(a1, a2, 8 * h + 4, ws, n1, d1);
+        vs = fmaf(fmaf(n0, d1, n1 * d0), __builtin_amdgcn_rcpf(d0 * d1), vs);
+      }
+    }
+    if constexpr (SCALED) vs *= inv_acc;
+    ws += __shfl_xor(ws, 32);
+    vs += __shfl_xor(vs, 32);
+  };
+
+  NTile<WH, AXIS, NYS> ht;
+  auto issue = [&](int id) {
+    const PassTile q = pass_tile<NYS>(p, id);
+    ht.issue(frame_src(p, q.f), p.src_dt, q.sp, 0, q.W, q.H, q.x0 + G::OX, q.y0 + G::OY);
+  };
+  if (static_cast<int>(blockIdx.x) < total_tiles) issue(xcd_tile(blockIdx.x, total_tiles));
+  int buf = 0;
+  for (int L = blockIdx.x; L < total_tiles; L += gridDim.x, buf ^= 1) {
+    const PassTile pt = pass_tile<NYS>(p, xcd_tile(L, total_tiles));
+    const int f = pt.f;
+    const int y0 = pt.y0, x0 = pt.x0;
+    // the other buffer was last read two tiles ago, before the barrier below was last
+    // passed by every wave
+    ht.commit(tile[buf], p.src_dt, p.src_max, p.src_rmax);
+    __syncthreads();
+    if (L + static_cast<int>(gridDim.x) < total_tiles) issue(xcd_tile(L + gridDim.x, total_tiles));
+
+#pragma unroll 1
+    for (int sub = 0; sub < NYS / NY; ++sub) {
+      const int tx = threadIdx.x % NX, ty = threadIdx.x / NX + sub * NY;
+      float m, sd, orig;
+      {
+        const float* tl = tile[buf];
+        float xv[K];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int b = 0; b < WH; ++b) xv[a * WH + b] = tl[G::at(tx, ty, a, b)];
+        orig = tl[G::centre(tx, ty)];
+        float is;
+        normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
+        M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED, is);
+      }
+
+      float wsum = 0.f, vsum = 0.f;
+      if constexpr (NSL == 1) {
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          float ws, vs;
+          group_sums(wave * 2 + j, ws, vs);
+          if (half == j) {
+            wsum = ws;
+            vsum = vs;
+          }
+        }
+      } else {
+        // (also orders the previous sub-tile's reads of `part` before this one's writes)
+        __syncthreads();
+        for (int g = pslice; g < NGROUP; g += PSL) {
+          float ws, vs;
+          group_sums(g, ws, vs);
+          if (half == 0) {
+            part[nslice][g * 32 + col][0] = ws;
+            part[nslice][g * 32 + col][1] = vs;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NSL; ++s) {
+          wsum += part[s][threadIdx.x][0];
+          vsum += part[s][threadIdx.x][1];
+        }
+      }
+
+      const int x = x0 + tx, y = y0 + ty;
+      if (x < pt.W && y < pt.H) {
+        const float pred = fminf(fmaxf(m + 5.0f * vsum * __builtin_amdgcn_rcpf(wsum) * sd, 0.f), 1.f);
+        store_result<AXIS>(p, frame_dst(p, f), pt.dp, x, y, orig, pred);
+      }
+      if constexpr (NSL == 1) wave_lds_sync();
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------- launcher
 
 template <int WH, int AXIS>
@@ -644,6 +881,55 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
     case 16: MPX_NN(FM, TERMS, 4, 4, SC) break;     \
     default: return hipErrorInvalidValue;           \
   }
+  // 32x32x16 tiles for the F16 / BF16 forms with nns >= 32 when they pay (measured, 8 x
+  // 1080p): win8x6 (K = 48 in three chunks of 16 instead of two of 32 half padded; nns256
+  // FP16 +8 %, BF16 +25 %, nns128 FP16 +61 %) and the three-product win8x4 forms (BF16
+  // nns128 +13 %); the one-product win8x4 forms keep 16x16x32 (K = 32 unpadded there, and
+  // the 32x32 form measured 3-9 % slower)
+#define MPX_NN32(FM, TERMS, NSL, TPW, SC)                                                             \
+  {                                                                                                   \
+    auto kern = nnedi3_mfma32<WH, AXIS, FM, TERMS, NSL, TPW, SC>;                                     \
+    static std::atomic<int> occ{0};                                                                   \
+    int per_cu = occ.load(std::memory_order_relaxed);                                                 \
+    if (!per_cu) {                                                                                    \
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) \
+        per_cu = 1;                                                                                   \
+      occ.store(per_cu, std::memory_order_relaxed);                                                   \
+    }                                                                                                 \
+    const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
+    const int grid = min(total, device_cu_count() * k);                                               \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.bias, nw.xscale, inv_acc, total); \
+  }
+#define MPX_NN32_NT(FM, TERMS, SC)                  \
+  switch (nw.nns / 32) {                            \
+    case 1: MPX_NN32(FM, TERMS, 1, 1, SC) break;    \
+    case 2:                                         \
+      if (nw.split) MPX_NN32(FM, TERMS, 2, 1, SC)   \
+      else MPX_NN32(FM, TERMS, 1, 2, SC)            \
+      break;                                        \
+    case 4:                                         \
+      if (nw.split) MPX_NN32(FM, TERMS, 4, 1, SC)   \
+      else MPX_NN32(FM, TERMS, 2, 2, SC)            \
+      break;                                        \
+    case 8: MPX_NN32(FM, TERMS, 4, 2, SC) break;    \
+    default: return hipErrorInvalidValue;           \
+  }
+  {
+    const bool three = nw.precision == 2 || nw.precision == 4;
+    if (nw.nns >= 32 && (three || (WH == 6 && nw.precision == 1))) {
+      if (nw.precision == 1) {
+        if (nw.nns == 128) MPX_NN32_NT(F_F16, 2, false)
+        else MPX_NN32_NT(F_F16, 1, false)
+      } else if (nw.precision == 4) {
+        MPX_NN32_NT(F_F16, 3, true)
+      } else {
+        MPX_NN32_NT(F_BF16, 3, false)
+      }
+      return hipGetLastError();
+    }
+  }
+#undef MPX_NN32_NT
+#undef MPX_NN32
   if (nw.precision == 1) {
     // FP16 form: one product, except nns 128, whose weights make the window's FP16
     // rounding alone exceed 2^-10 max-abs on some inputs (1.2e-3 measured): there the
