@@ -453,6 +453,10 @@ def main():
             err = parity_record("nnedi3-nns256-win8x6 %s 1920x1080 10-bit b8" % p)
             if err:
                 rec["error_vs_fp64_oracle"] = err
+            cf = os.path.join(ROOT, "profiles", "counters_nnedi3_nns256_8x1080p.json")
+            if p == "fp16" and os.path.exists(cf):
+                with open(cf) as f:
+                    rec["mfma_busy_frac"] = json.load(f)["mfma_busy_frac"]   # hardware counters, same workload
             # FP8 (e4m3 weights and window) is slower than FP16 on this kernel and ~40 dB
             # less accurate: reported for its accuracy only
             (accuracy_forms if p == "fp8" else secondary)["nnedi3-nns256-win8x6 %s b8" % p] = rec
