@@ -319,6 +319,15 @@ int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int targ
     p->scratch_bytes = static_cast<size_t>(p->scratch_frame) * 4 * max_batch;
     rc = ctx->alloc(p->scratch_bytes, &p->scratch);
     if (rc) return rc;
+    mpx_host::DeviceScope dscope;
+    if (int drc = dscope.enter(ctx->device)) {
+      ctx->release(p->scratch, p->scratch_bytes);
+      return drc;
+    }
+    if (hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess) {
+      ctx->release(p->scratch, p->scratch_bytes);
+      return fail(MPX_ERR_HIP, "hipEventCreateWithFlags failed");
+    }
   }
   *out = p.release();
   return applies ? MPX_OK : MPX_SKIPPED;
@@ -345,6 +354,11 @@ int64_t mpx_plan_bucket_count(const mpx_plan* plan) {
 void mpx_plan_destroy(mpx_plan* plan) {
   if (!plan) return;
   if (plan->scratch) plan->ctx->release(plan->scratch, plan->scratch_bytes);
+  if (plan->done) {
+    mpx_host::DeviceScope dscope;
+    (void)dscope.enter(plan->ctx->device);
+    (void)hipEventDestroy(plan->done);
+  }
   mpx::zoom_plan_destroy(plan->ctx, plan);
   delete plan;
 }
@@ -494,7 +508,33 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
   return dispatch(plan, d, static_cast<hipStream_t>(hip_stream), plan->scratch);
 }
 
+static int launch_all(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* interm);
+
+// The plan's own intermediates serve one stream at a time.  A run on another stream while
+// the last run's work may still be in flight would race on them: it is refused before
+// anything is enqueued.  Runs captured into a graph are not tracked (the graph's launches
+// are ordered by the caller); mpx_run_static's array is the caller's to serialise.
 static int dispatch(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* interm) {
+  if (!plan->done || interm != plan->scratch) return launch_all(plan, d, st, interm);
+  std::lock_guard<std::mutex> lk(plan->run_mu);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) return fail(MPX_ERR_HIP, "hipStreamIsCapturing failed");
+  if (cs != hipStreamCaptureStatusNone) return launch_all(plan, d, st, interm);
+  if (plan->ran && st != plan->last) {
+    const hipError_t q = hipEventQuery(plan->done);
+    if (q == hipErrorNotReady)
+      return fail(MPX_ERR_ARG, "the plan's intermediates are in use by its previous run on another stream: use one "
+                               "plan per stream, or wait for that stream first");
+    if (q != hipSuccess) return fail(MPX_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+  }
+  if (int rc = launch_all(plan, d, st, interm)) return rc;
+  HIP_TRY(hipEventRecord(plan->done, st));
+  plan->last = st;
+  plan->ran = true;
+  return MPX_OK;
+}
+
+static int launch_all(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* interm) {
   const mpx_variant_info& inf = plan->info;
   hipError_t e = hipSuccess;
   if (!inf.applies) {

@@ -183,5 +183,11 @@ struct mpx_plan {
   void* scratch = nullptr;            // intermediates, max_batch frames
   size_t scratch_bytes = 0;
   int64_t scratch_frame = 0;          // elements per frame
+  // the intermediates serve one stream at a time: the last stream that ran the plan and
+  // an event recorded after its work (plans with intermediates only)
+  std::mutex run_mu;
+  hipEvent_t done = nullptr;
+  hipStream_t last = nullptr;
+  bool ran = false;
 };
 

@@ -41,10 +41,31 @@ def test_native_and_main_routing():
     out, offs, ran = apply_hook("ravu-r2-yuv", fr444, (96, 64))
     want = mpx.upscale("ravu-r2-yuv", torch.stack([fr444["Y"], fr444["U"], fr444["V"]]))
     assert ran and torch.equal(out["V"], want[2])
-    with pytest.raises(ValueError):
-        apply_hook("ravu-r2-yuv", _yuv(48, 32), (96, 64))                 # 4:2:0 into NATIVE
     rgb = {"RGB": torch.stack([fr444["Y"], fr444["U"], fr444["V"]])}
     out, offs, ran = apply_hook("ravu-zoom-ar-r2-rgb", rgb, (72, 48))
     assert ran and tuple(out["RGB"].shape) == (3, 48, 72)
     out, offs, ran = apply_hook("ravu-lite-r2", rgb, (96, 64))             # no LUMA in RGB
     assert not ran
+
+
+@pytest.mark.parametrize("loc", ["left", "center"])
+def test_native_on_420_after_cscale_vs_oracle(loc):
+    """4:2:0 into a NATIVE hook: chroma resampled to the luma size (bilinear, the given
+    siting), then the three planes through ravu-r2-yuv -- against the FP64 oracle's
+    resampler and RAVU on the same planes."""
+    from mpx.hooks import apply_hook, chroma_offset
+    from oracle.ravu import ravu
+    from oracle.resample import resample
+    w, h = 50, 34
+    fr = _yuv(w, h, seed=9)
+    out, offs, ran = apply_hook("ravu-r2-yuv", fr, (2 * w, 2 * h), chroma_loc=loc)
+    assert ran and offs["U"] == (-0.5, -0.5)
+    planes = [fr["Y"].cpu().numpy().astype(np.float64)]
+    for k in ("U", "V"):
+        c = fr[k].cpu().numpy().astype(np.float64)
+        planes.append(resample(c, w, h, offset=chroma_offset((w, h), c.shape[::-1], loc), name="bilinear"))
+    ref, _ = ravu(np.stack(planes), 2, channels="yuv")
+    got = np.stack([out[k].cpu().numpy() for k in ("Y", "U", "V")])
+    err = float(np.abs(got - ref).max())
+    print("ravu-r2-yuv on 4:2:0 (%s siting) vs FP64 oracle: max-abs %.3g" % (loc, err))
+    assert err <= 5e-5

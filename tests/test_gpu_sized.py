@@ -70,3 +70,34 @@ def test_frame_planes_checked_before_launch(name, planes):
         assert srv.launches == 0
     finally:
         srv.close()
+
+
+def test_plan_refuses_another_stream_while_its_intermediates_are_busy():
+    """A plan with intermediates (NNEDI3 doubling both axes) runs long work on stream A; a
+    run of the same plan on stream B before A finishes is refused (MPX_ERR_ARG) without
+    enqueuing anything; after A is waited for, B runs and equals A's result."""
+    import mpx
+    from mpx.frames import make_plane
+    w, h, b = 1920, 1080, 4
+    ctx = mpx.Context(0)
+    plan = ctx.plan("nnedi3-nns256-win8x6", w, h, in_dtype=torch.float16, out_dtype=torch.float32,
+                    precision="fp32", max_batch=b)
+    x = torch.from_numpy(np.stack([make_plane(w, h, seed=70 + i, bits=10) for i in range(b)])
+                         .astype(np.float16)).cuda()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    oa = torch.empty((b, 2 * h, 2 * w), device="cuda")
+    ob = torch.full((1, 2 * h, 2 * w), -1.0, device="cuda")
+    torch.cuda.synchronize()
+    print("long FP32 nns256 run on stream A, then the same plan on stream B", flush=True)
+    plan.run(x, out=oa, stream=sa)            # ~17 ms of work: still in flight below
+    with pytest.raises(mpx.MpxError) as ei:
+        plan.run(x[:1], out=ob, stream=sb)
+    assert ei.value.code == -1 and "another stream" in str(ei.value)
+    sa.synchronize()
+    sb.synchronize()
+    assert bool((ob == -1.0).all())           # the refused run enqueued nothing
+    plan.run(x[:1], out=ob, stream=sb)        # A is done: B may take the plan
+    sb.synchronize()
+    assert torch.equal(ob[0], oa[0])
+    plan.close()
+    ctx.close()

@@ -47,3 +47,24 @@ def test_kernel_values():
     assert abs(kernel("mitchell", 0.0) - 8 / 9) < 1e-15
     assert abs(kernel("spline36", 0.0) - 1.0) < 1e-15 and abs(kernel("spline36", 1.0)) < 1e-15
     assert abs(kernel("lanczos3", 0.0) - 1.0) < 1e-15 and abs(kernel("lanczos3", 2.0)) < 1e-15
+
+
+@pytest.mark.parametrize("loc,sub", [("left", 2), ("center", 2), ("left", 4), ("center", 3)])
+def test_chroma_siting_offset_reproduces_a_ramp(loc, sub):
+    """A chroma plane whose samples hold their own luma x position (left siting: chroma k
+    on luma sample sub*k; centre siting: at the centre of luma samples sub*k..sub*k+sub-1,
+    luma position sub*k + (sub-1)/2) resampled bilinearly to the luma width with the
+    siting's offset gives back the luma positions 0, 1, 2, ... away from the clamped
+    edges: the offset places every chroma sample where its siting says."""
+    from mpx.hooks import chroma_offset
+    from oracle.resample import resample
+    cw, lw = 12, 12 * sub
+    k = np.arange(cw, dtype=np.float64)
+    pos = sub * k if loc == "left" else sub * k + (sub - 1) / 2.0
+    c = np.tile(pos, (3, 1))
+    off = chroma_offset((lw, 3), (cw, 3), loc)
+    assert off[1] == 0.0
+    up = resample(c, lw, 3, offset=off, name="bilinear")[1]
+    j = np.arange(lw, dtype=np.float64)
+    inner = (j >= pos[0]) & (j <= pos[-1])
+    assert np.allclose(up[inner], j[inner], atol=1e-12)
