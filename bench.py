@@ -25,6 +25,7 @@ HEADLINE = "nnedi3-nns32-win8x4"
 W, H = 1920, 1080
 PEAK_FP16 = 2500.0  # TFLOP/s, dense FP16 MFMA (MI355X)
 PEAK_HBM = 8000.0   # GB/s
+PEAK_LDS = 256 * 128 * 2.4  # GB/s: 128 bytes per clock per CU at 2.4 GHz
 
 
 # (variant, input size, target, planes, batch, dtype name) of the RAVU secondary lines
@@ -68,6 +69,17 @@ def zoom_ar3_weights_dir(tmp):
     ext.write_bin(os.path.join(tmp, "ravu_zoom_lut3_ar.bin"),
                   np.abs(ext.read_bin(os.path.join(wdir, "ravu_zoom_lut3.bin"))).astype(np.float32))
     return tmp
+
+
+def lds_bytes_per_frame(key):
+    """LDS bytes one frame of a RAVU secondary workload moves (hardware counters,
+    profiles/counters_ravu_lds.json), or None."""
+    path = os.path.join(ROOT, "profiles", "counters_ravu_lds.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        w = json.load(f)["workloads"].get(key)
+    return w["lds_bytes_per_frame"] if w else None
 
 
 def nnedi3_flop_per_px(name):
@@ -475,6 +487,11 @@ def main():
                 "mpx_s": round(b * 10 * opx / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
                 "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
                 "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4)}
+            lds = lds_bytes_per_frame(key)
+            if lds:
+                # LDS traffic per frame from the hardware counters of the same workload
+                secondary[key]["lds_gb_s"] = round(b * 10 * lds / wall / 1e9, 1)
+                secondary[key]["lds_frac"] = round(b * 10 * lds / wall / 1e9 / PEAK_LDS, 4)
 
         # ravu-zoom-ar-r3-rgb at 2x and 1.5x on a supplied (synthetic) anti-ringing table
         import tempfile
