@@ -41,6 +41,12 @@ extern "C" {
 #define MPX_PREC_BF16 2   /* BF16 MFMA, high/low split (three products), FP32 accumulation */
 #define MPX_PREC_FP8 3    /* FP8 e4m3 MFMA, per-neuron power-of-two scales, FP32 accum.   */
 #define MPX_PREC_FP16X3 4 /* FP16 MFMA, high/low split (three products), FP32 accumulation */
+/* OR'ed into `precision` of mpx_plan_create: store the intermediates (NNEDI3's double_y
+ * plane, RAVU's int11 plane) in FP16, round to nearest even, as mpv stores a hook's SAVEd
+ * textures and pass outputs in its default rgba16f FBOs (e.g. nnedi3-nns256-win8x6.hook:18,
+ * ravu-r3.hook:19).  With MPX_F16 output the plan then reproduces mpv's default storage;
+ * without the flag the intermediates stay FP32 (closer to the FP64 shader math). */
+#define MPX_STORE_FP16 0x100
 
 /* ---- variant families / channel forms --------------------------------------------- */
 #define MPX_FAMILY_NNEDI3 0

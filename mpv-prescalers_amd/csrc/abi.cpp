@@ -272,6 +272,8 @@ int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int targ
   if (max_batch < 1 || max_batch * (v->channels == MPX_CH_LUMA ? 1 : 3) > 65535)
     return fail(MPX_ERR_SHAPE, "max_batch out of range");
   if (in_type == MPX_U16 && (bits < 1 || bits > 16)) return fail(MPX_ERR_ARG, "bits must be 1..16 for MPX_U16");
+  const bool store16 = (precision & MPX_STORE_FP16) != 0;
+  precision &= ~MPX_STORE_FP16;
   if (precision < MPX_PREC_FP32 || precision > MPX_PREC_FP16X3) return fail(MPX_ERR_ARG, "bad precision");
   if (v->family != MPX_FAMILY_NNEDI3 && precision != MPX_PREC_FP32)
     return fail(MPX_ERR_UNSUPPORTED, "RAVU variants compute in FP32 only");
@@ -285,6 +287,7 @@ int mpx_plan_create(mpx_ctx* ctx, const char* name, int in_w, int in_h, int targ
   p->in_type = in_type;
   p->out_type = out_type;
   p->precision = precision;
+  p->store16 = store16 ? 1 : 0;
   p->max_batch = max_batch;
   p->in_max = in_type == MPX_U8 ? 255.f : (in_type == MPX_U16 ? static_cast<float>((1u << bits) - 1u) : 1.f);
   const bool applies = p->info.applies != 0;
@@ -419,6 +422,7 @@ int mpx_host::run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* i
   d.out_dt = plan->out_type;
   d.in_max = plan->in_max;
   d.in_rmax = 1.0f / d.in_max;
+  d.store16 = plan->store16;
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = io->batch;
@@ -457,6 +461,7 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
   d.out_dt = plan->out_type;
   d.in_max = plan->in_max;
   d.in_rmax = 1.0f / d.in_max;
+  d.store16 = plan->store16;
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = n;
@@ -502,6 +507,7 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
   d.out_dt = plan->out_type;
   d.in_max = plan->in_max;
   d.in_rmax = 1.0f / d.in_max;
+  d.store16 = plan->store16;
   d.W = plan->in_w;
   d.H = plan->in_h;
   d.batch = n;

@@ -40,6 +40,9 @@ struct Io {
   int64_t fin_pitch[kMaxTab], fout_pitch[kMaxTab];
   int tiles[kMaxTab + 1];
   int sx, sy;         // output scale of the launch (for the output plane stride)
+  // intermediates (NNEDI3's double_y plane, RAVU's int11) rounded to FP16 as mpv's
+  // default rgba16f FBOs store them (MPX_STORE_FP16)
+  int store16;
 };
 
 // One tile of a launch: its frame, origin and that frame's geometry.

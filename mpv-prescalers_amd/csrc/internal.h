@@ -175,6 +175,7 @@ struct mpx_plan {
   const mpx_host::Variant* v = nullptr;
   mpx_variant_info info{};
   int in_w = 0, in_h = 0, in_type = 0, out_type = 0, precision = 0, max_batch = 0;
+  int store16 = 0;                    // MPX_STORE_FP16: FP16 intermediates
   float in_max = 1.f;
   const __half* lut = nullptr;        // main LUT (RAVU family)
   const __half* lut_ar = nullptr;     // zoom-ar LUT

@@ -980,7 +980,7 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
       p.dst = tmp;
       p.dst_pitch = io.W;
       p.dst_bstride = tmp_frame;
-      p.dst_dt = DT_F32;
+      p.dst_dt = io.store16 ? DT_F16 : DT_F32;
     } else {
       p.dst = io.out;
       p.dst_pitch = io.out_pitch;
@@ -1007,7 +1007,7 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
       p.src = tmp;
       p.src_pitch = io.W;
       p.src_bstride = tmp_frame;
-      p.src_dt = DT_F32;
+      p.src_dt = io.store16 ? DT_F16 : DT_F32;
       p.src_max = 1.f;
       p.src_rmax = 1.f;
       p.H = 2 * io.H;

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(RF_T) __attribute__((amdgpu_waves_per_eu(6))) void 
   const int64_t plane = static_cast<int64_t>(ft.W) * ft.H;
   float* o = int11 + f * int11_frame + static_cast<int64_t>(y) * ft.W + x;
 #pragma unroll
-  for (int c = 0; c < P; ++c) o[c * plane] = res[c];
+  for (int c = 0; c < P; ++c) o[c * plane] = io.store16 ? static_cast<float>(static_cast<_Float16>(res[c])) : res[c];
 }
 
 // Lattice position of sample (X, Y) of step 2 (int10, centre (x+1/2, y)) / step 3

@@ -17,6 +17,7 @@ STATUS = {
 }
 U8, U16, F16, F32 = 0, 1, 2, 3
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "fp8": 3, "fp16x3": 4}
+STORE_FP16 = 0x100   # MPX_STORE_FP16, OR'ed into the precision
 FILTER = {"bilinear": 0, "catmull_rom": 1, "mitchell": 2, "spline36": 3, "lanczos3": 4}
 FAMILY = {0: "nnedi3", 1: "ravu", 2: "ravu-lite", 3: "ravu-3x", 4: "ravu-zoom"}
 CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}

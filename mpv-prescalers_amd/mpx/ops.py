@@ -41,16 +41,20 @@ class Plan:
     ``max_batch`` frames).  Not re-entrant: one plan serves one stream at a time."""
 
     def __init__(self, ctx, name, in_w, in_h, target_w, target_h, in_dtype, out_dtype, precision="fp32",
-                 max_batch=1, bits=10):
+                 max_batch=1, bits=10, storage="fp32"):
         self.ctx = ctx
         self.name = name
         self.in_w, self.in_h = in_w, in_h
         self.in_dtype, self.out_dtype = in_dtype, out_dtype
         self.max_batch = max_batch
         self.precision = precision
+        if storage not in ("fp32", "fp16"):
+            raise ValueError("storage must be 'fp32' or 'fp16'")
+        self.storage = storage
         h = ctypes.c_void_p()
+        prec = _abi.PREC[precision] | (_abi.STORE_FP16 if storage == "fp16" else 0)
         rc = LIB.mpx_plan_create(ctx._h, name.encode(), in_w, in_h, target_w, target_h, _DT[in_dtype], bits,
-                                 _DT[out_dtype], _abi.PREC[precision], max_batch, ctypes.byref(h))
+                                 _DT[out_dtype], prec, max_batch, ctypes.byref(h))
         check(rc)
         self._h = h
         self.skipped = rc == _abi.MPX_SKIPPED
@@ -250,18 +254,20 @@ class Context:
         return LIB.mpx_ctx_bytes(self._h)
 
     def plan(self, name, in_w, in_h, target=None, in_dtype=torch.float32, out_dtype=torch.float32,
-             precision="fp32", max_batch=1, bits=10):
+             precision="fp32", max_batch=1, bits=10, storage="fp32"):
+        """``storage="fp16"``: intermediates stored in FP16 as mpv's default FBOs store
+        them (with ``out_dtype=torch.float16`` the plan reproduces mpv's default storage)."""
         tw, th = target if target is not None else default_target(name, in_w, in_h)
-        return Plan(self, name, in_w, in_h, tw, th, in_dtype, out_dtype, precision, max_batch, bits)
+        return Plan(self, name, in_w, in_h, tw, th, in_dtype, out_dtype, precision, max_batch, bits, storage)
 
-    def cached_plan(self, name, in_w, in_h, target, in_dtype, out_dtype, precision, batch, bits):
-        key = (name, in_w, in_h, target, in_dtype, out_dtype, precision, bits)
+    def cached_plan(self, name, in_w, in_h, target, in_dtype, out_dtype, precision, batch, bits, storage="fp32"):
+        key = (name, in_w, in_h, target, in_dtype, out_dtype, precision, bits, storage)
         with self._lock:
             p = self._plans.get(key)
             if p is None or p.max_batch < batch:
                 if p is not None:
                     p.close()
-                p = self.plan(name, in_w, in_h, target, in_dtype, out_dtype, precision, max(batch, 1), bits)
+                p = self.plan(name, in_w, in_h, target, in_dtype, out_dtype, precision, max(batch, 1), bits, storage)
                 self._plans[key] = p
             return p
 
@@ -292,12 +298,14 @@ def default_context(device=None):
         return _ctx[dev]
 
 
-def upscale(name, x, target=None, precision="fp32", out_dtype=None, bits=10, buckets=False, stream=None):
+def upscale(name, x, target=None, precision="fp32", out_dtype=None, bits=10, buckets=False, stream=None,
+            storage="fp32"):
     """Run variant ``name`` on device tensor ``x``.
 
     ``target`` = (width, height) the video is displayed at (mpv's OUTPUT size): it gates
     the hook like //!WHEN and, for ravu-zoom, is the output size.  Default: 2x (3x for
-    ravu-3x).  ``out_dtype`` defaults to the input dtype for float inputs, float32 else."""
+    ravu-3x).  ``out_dtype`` defaults to the input dtype for float inputs, float32 else.
+    ``storage="fp16"`` stores the intermediates in FP16 as mpv's default FBOs do."""
     planes3 = name.endswith("-yuv") or name.endswith("-rgb")
     in_h, in_w = x.shape[-2:]
     if out_dtype is None:
@@ -306,7 +314,8 @@ def upscale(name, x, target=None, precision="fp32", out_dtype=None, bits=10, buc
     if x.dim() == (4 if planes3 else 3):
         batch = x.shape[0]
     tgt = tuple(target) if target is not None else default_target(name, in_w, in_h)
-    plan = default_context(x.device).cached_plan(name, in_w, in_h, tgt, x.dtype, out_dtype, precision, batch, bits)
+    plan = default_context(x.device).cached_plan(name, in_w, in_h, tgt, x.dtype, out_dtype, precision, batch, bits,
+                                                 storage)
     return plan.run(x, stream=stream, buckets=buckets)
 
 

@@ -87,11 +87,23 @@ def double_x(plane, nns, wh):
     return double_y(np.ascontiguousarray(plane.T), nns, wh).T
 
 
-def nnedi3(plane, nns, wh, axes="yx"):
-    """Full hook: double_y (if 'y' in axes) then double_x (if 'x' in axes)."""
+def _f16(a):
+    """FP16 storage (round to nearest even) of an FBO-stored texture."""
+    return np.asarray(a).astype(np.float16).astype(np.float64)
+
+
+def nnedi3(plane, nns, wh, axes="yx", fbo16=False):
+    """Full hook: double_y (if 'y' in axes) then double_x (if 'x' in axes).
+    fbo16: each combine pass's output (the hooked plane: kept rows and the SAVEd
+    predictions, nnedi3-nns256-win8x6.hook:18,346-360) is stored in FP16, as mpv's
+    default rgba16f FBOs store it, and the next pass reads those values."""
     p = np.asarray(plane, dtype=np.float64)
     if "y" in axes:
         p = double_y(p, nns, wh)
+        if fbo16:
+            p = _f16(p)
     if "x" in axes:
         p = double_x(p, nns, wh)
+        if fbo16:
+            p = _f16(p)
     return p

@@ -55,7 +55,12 @@ def _ar_clamp(res, taps, wgt_pairs, strength=0.8):
 
 # ----------------------------------------------------------------------------- ravu-lite
 
-def ravu_lite(img, r, ar=False, channels="luma"):
+def _f16(a):
+    """FP16 storage (round to nearest even) of an FBO-stored texture."""
+    return np.asarray(a).astype(np.float16).astype(np.float64)
+
+
+def ravu_lite(img, r, ar=False, channels="luma", fbo16=False):
     """RAVU-lite 2x (ravu-lite-r3.hook:22-131, ravu-lite-ar-r3.hook:22-196).
 
     n = 2r-1 window at offsets -(r-1)..(r-1); LUT texel i holds the 4 sub-pixel
@@ -94,6 +99,8 @@ def ravu_lite(img, r, ar=False, channels="luma"):
         res = _ar_clamp(res, taps, wgts)
     else:
         res = np.clip(res, 0.0, 1.0)
+    if fbo16:   # ravu_lite_int (ravu-lite-r3.hook:19-21) and the step-2 output in FP16
+        res = _f16(res)
     H, Wd = planes.shape[-2:]
     out = np.empty((2 * H, 2 * Wd))
     for dx in range(2):
@@ -104,7 +111,7 @@ def ravu_lite(img, r, ar=False, channels="luma"):
 
 # ----------------------------------------------------------------------------- ravu-3x
 
-def ravu_3x(img, r, channels="luma"):
+def ravu_3x(img, r, channels="luma", fbo16=False):
     """RAVU-3x (compute/ravu-3x-r3.hook:22-163).  LUT texels 2i, 2i+1 hold (w0, w1) of
     tap i; res0 += l_i*w0 + l_j*w1.wzyx, res1 += l_i*w1 + l_j*w0.wzyx (j = n^2-1-i);
     output k = 3*dx+dy: k<4 from res0, k=4 the input pixel, k>4 from res1."""
@@ -142,6 +149,8 @@ def ravu_3x(img, r, channels="luma"):
             else:
                 v = res1[..., k - 5]
             out[:, dy::3, dx::3] = v
+    if fbo16:   # imageStore into the hook's FP16 output (compute/ravu-3x-r3.hook:158-162)
+        out = _f16(out)
     return _out(out, channels), {"row": row, "diag": diag}
 
 
@@ -182,7 +191,7 @@ def _lattice_samples(planes, int11, channels, n, step):
     return samp, lum
 
 
-def ravu(img, r, channels="luma"):
+def ravu(img, r, channels="luma", fbo16=False):
     """RAVU 2x in three predictions + combine (ravu-r3.hook:15-437).
 
     step1: int11 at (x+1/2, y+1/2) from the n = 2r window at offsets -(r-1)..r;
@@ -198,6 +207,8 @@ def ravu(img, r, channels="luma"):
     ang, st, co, diag11 = hash_abd(a, b, d, STRENGTH_LOG2)
     rows["int11"] = lut_row(ang, st, co, 9, lut.shape[0])
     int11 = _ravu_taps(samp, lut[rows["int11"]], n)
+    if fbo16:   # SAVE ravu_int11 (ravu-r3.hook:19) in an FP16 texture, read by steps 2 and 3
+        int11 = _f16(int11)
     outs = {}
     diags = {"int11": diag11}
     for step, key in ((2, "int10"), (3, "int01")):
@@ -213,6 +224,8 @@ def ravu(img, r, channels="luma"):
     out[:, 1::2, 0::2] = outs["int01"]
     out[:, 0::2, 1::2] = outs["int10"]
     out[:, 1::2, 1::2] = int11
+    if fbo16:   # the combine pass's output FBO (the kept pixels are stored too)
+        out = _f16(out)
     return _out(out, channels), {"row": rows, "diag": diags}
 
 
@@ -240,7 +253,7 @@ def _bilinear_lut(lut, group, tx, ty):
     return (v00 * (1 - fx) + v01 * fx) * (1 - fy) + (v10 * (1 - fx) + v11 * fx) * fy
 
 
-def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None, cols=None, rows=None):
+def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None, cols=None, rows=None, fbo16=False):
     """RAVU-zoom(-ar) to an arbitrary size (ravu-zoom-r2.hook:22-134,
     ravu-zoom-ar-r2-rgb.hook:25-224).
 
@@ -312,4 +325,6 @@ def ravu_zoom(img, r, out_w, out_h, ar=False, channels="luma", ar_lut=None, cols
         res = _ar_clamp(res, taps, [wts[p] for p in range(n * n)])
     else:
         res = np.clip(res, 0.0, 1.0)
+    if fbo16:   # the hook's output texture in FP16
+        res = _f16(res)
     return _out(res, channels), {"row": bucket, "diag": diag}

@@ -601,9 +601,12 @@ def _loop_end():
     Shared.active = True
 
 
-def run_hook(path, planes, target=None):
+def run_hook(path, planes, target=None, fbo16=False):
     """Run every pass of the hook file on `planes` ((H,W) luma or (3,H,W)); returns the
-    hooked plane(s) after the last pass.  `target` = (w, h) of mpv's OUTPUT."""
+    hooked plane(s) after the last pass.  `target` = (w, h) of mpv's OUTPUT.
+    fbo16: every pass's output texture (SAVE or the replaced hooked plane) is stored in
+    FP16 (round to nearest even), as mpv's default rgba16f FBOs store it; the source
+    planes themselves are not rounded."""
     passes, luts = parse_hook_file(path)
     img = np.asarray(planes, dtype=np.float64)
     if img.ndim == 2:
@@ -721,6 +724,8 @@ def run_hook(path, planes, target=None):
             r = g["hook"]()
             for k in range(4):
                 result[..., k] = np.broadcast_to(np.asarray(r.c[k], dtype=np.float64), xx.shape).reshape(out_h, out_w)
+        if fbo16:
+            result = result.astype(np.float16).astype(np.float64)
         tex = Tex(result)
         if "SAVE" in p:
             saved[p["SAVE"]] = tex

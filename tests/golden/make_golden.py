@@ -8,6 +8,8 @@ ravu-3x, which exists only there) is stored with its input in ``hooks_golden.npz
   <name>/target  [target_w, target_h] the hook was run for
   <name>/out     float64 output planes
   <name>/flavours  names of the flavours that were executed and agreed
+  <name>/out16   float64 output of the root (compute/) flavour with every pass's output
+                 texture stored in FP16, as mpv's default rgba16f FBOs do
 Run in the container holding the reference:  python tests/golden/make_golden.py
 """
 import json
@@ -77,6 +79,8 @@ def main():
         data[name + "/target"] = np.array(tgt)
         data[name + "/out"] = base
         data[name + "/flavours"] = np.array(keys)
+        path = os.path.join(REF, ("" if "root" in outs else keys[0]) + name + ".hook")
+        data[name + "/out16"] = hookexec.run_hook(path, img, tgt, fbo16=True)
         summary[name] = keys
         print("%-26s %-28s out %s" % (name, ",".join(keys), base.shape), flush=True)
     np.savez_compressed(OUT, **data)

@@ -13,22 +13,22 @@ _G = np.load(GOLD) if os.path.exists(GOLD) else None
 NAMES = sorted({k.split("/")[0] for k in _G.files}) if _G is not None else []
 
 
-def run_oracle(name, img, target):
+def run_oracle(name, img, target, fbo16=False):
     from oracle.nnedi3 import nnedi3
     from oracle.ravu import ravu, ravu_3x, ravu_lite, ravu_zoom
     ch = "yuv" if name.endswith("-yuv") else ("rgb" if name.endswith("-rgb") else "luma")
     if name.startswith("nnedi3"):
         nns = int(name.split("nns")[1].split("-")[0])
-        return nnedi3(img, nns, int(name[-1]))
+        return nnedi3(img, nns, int(name[-1]), fbo16=fbo16)
     import re
     r = int(re.search(r"-r(\d)", name).group(1))
     if name.startswith("ravu-lite"):
-        return ravu_lite(img, r, ar="-ar-" in name)[0]
+        return ravu_lite(img, r, ar="-ar-" in name, fbo16=fbo16)[0]
     if name.startswith("ravu-3x"):
-        return ravu_3x(img, r, channels=ch)[0]
+        return ravu_3x(img, r, channels=ch, fbo16=fbo16)[0]
     if name.startswith("ravu-zoom"):
-        return ravu_zoom(img, r, int(target[0]), int(target[1]), ar="-ar-" in name, channels=ch)[0]
-    return ravu(img, r, channels=ch)[0]
+        return ravu_zoom(img, r, int(target[0]), int(target[1]), ar="-ar-" in name, channels=ch, fbo16=fbo16)[0]
+    return ravu(img, r, channels=ch, fbo16=fbo16)[0]
 
 
 def test_golden_covers_every_shipped_variant():
@@ -49,6 +49,22 @@ def test_oracle_matches_reference_text(name):
     # sampler's coordinate arithmetic differ
     tol = 1e-12 if name.startswith("ravu-zoom") else 1e-13
     assert np.abs(got - want).max() <= tol
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_fp16_storage_matches_reference_text(name):
+    """The oracle's FP16-storage mode (mpv's default rgba16f FBOs: every SAVEd texture and
+    every pass's output rounded to FP16, read back by the next pass) against the
+    reference's passes executed with the same storage.  The FP64 evaluations differ by
+    <= 1e-12 before each rounding, so the rounded values agree unless one lands within
+    that of an FP16 rounding boundary (then by one FP16 step, carried on)."""
+    img = _G[name + "/in"]
+    tgt = _G[name + "/target"]
+    want = _G[name + "/out16"]
+    got = run_oracle(name, img, tgt, fbo16=True)
+    assert got.shape == want.shape
+    assert np.array_equal(got, got.astype(np.float16).astype(np.float64))   # FP16 values
+    assert np.abs(got - want).max() <= 1e-12
 
 
 def test_goldens_reproduce_from_reference_text(reference_dir):
