@@ -211,6 +211,7 @@ def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
         launches = srv.launches
     finally:
         srv.close()
+    native = _native_queue(reqs, prec, total_px, gaps)
     # (c) one frame per launch on one stream: the plain request loop (launch, wait for the
     # result, next request), the same launches issued back to back, and the plain request
     # loop with the plain forms (FP32 VALU NNEDI3)
@@ -227,7 +228,59 @@ def serving_bench(duration_s=2.0, rate=2000.0, seed=5):
             "speedup_vs_one_frame_per_launch_sync": round(sat / single_sync, 2),
             "speedup_vs_one_frame_per_launch_async": round(sat / single_async, 2),
             "speedup_vs_one_frame_per_launch_plain_fp32_sync": round(sat / plain_sync, 2),
-            "launches": launches, "small_frames": small}
+            "launches": launches, "small_frames": small,
+            "native_queue": dict(native, speedup_vs_one_frame_per_launch_sync=round(native["saturated_mpx_s"] / single_sync, 2),
+                                 speedup_vs_one_frame_per_launch_async=round(native["saturated_mpx_s"] / single_async, 2))}
+
+
+def _native_queue(reqs, prec, total_px, gaps):
+    """The same requests through the library's own queue (mpx_submit / mpx_poll: batching
+    in native threads): p50/p99 latency at the Poisson offered load (submit -> finish, as
+    the library saw it), then the saturated throughput (everything submitted under a hold,
+    released at once, at steady state: the burst runs once untimed first)."""
+    import numpy as np
+    import torch
+    from mpx.queue import Queue
+    from mpx.serving import percentile
+    q = Queue(hbm_budget=16 << 30, streams=4, max_batch=8)
+    try:
+        q.hold()   # warm-up burst: every (variant, stream) plan is made before timing
+        warm = [q.submit(v, x, precision=prec[v]) for v, x in reqs[:400]]
+        q.hold(False)
+        for t in warm:
+            t.wait(120)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        arr = t0 + np.cumsum(gaps)
+        tickets = []
+        for i, (v, x) in enumerate(reqs):
+            d = arr[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append((time.perf_counter(), q.submit(v, x, precision=prec[v])))
+        for _, t in tickets:
+            t.wait(300)
+            t.out = None
+        t_end = max(ts + t.latency for ts, t in tickets)
+        lat = [t.latency * 1e3 for _, t in tickets]
+        offered = {"p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+                   "mpx_s": round(total_px / (t_end - t0) / 1e6, 1), "frames": len(reqs),
+                   "achieved_fps": round(len(reqs) / (t_end - t0), 1)}
+        launches0 = q.launches
+        for rep in range(2):
+            q.hold()
+            t0 = time.perf_counter()
+            tickets = [(time.perf_counter(), q.submit(v, x, precision=prec[v])) for v, x in reqs]
+            q.hold(False)
+            for _, t in tickets:
+                t.wait(300)
+                t.out = None
+            dt = max(ts + t.latency for ts, t in tickets) - t0
+        launches = q.launches - launches0
+    finally:
+        q.close()
+    return {"offered_load": offered, "saturated_mpx_s": round(total_px / dt / 1e6, 1),
+            "saturated_launches_per_burst": launches // 2, "requests_per_burst": len(reqs)}
 
 
 def _burst(srv, reqs, prec):

@@ -220,6 +220,61 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
 #define MPX_STATIC_SCRATCH_BYTES (64u << 20)
 int mpx_run_static(mpx_plan* plan, const mpx_io* io, void* hip_stream);
 
+/* ---- request queue: continuous batching for hosts that serve many frames ----------- */
+#define MPX_PENDING 2             /* mpx_poll: the request has not finished yet          */
+#define MPX_ERR_DEADLINE (-9)     /* the request's deadline passed before it was launched */
+
+/* One frame to upscale.  `in` / `out` are device pointers owned by the caller and must
+ * stay valid until mpx_poll reports the request finished; planes are contiguous
+ * (in: pitch in_w, out: pitch out_w of mpx_query's answer for these sizes).  ready_stream:
+ * the hipStream_t that produces `in` (NULL: `in` is ready now); the launch waits for the
+ * work submitted to it before mpx_submit, without the host waiting.  deadline_us: a
+ * request still queued that long after mpx_submit fails with MPX_ERR_DEADLINE instead of
+ * running (0: no deadline). */
+typedef struct mpx_request {
+  const char* name;
+  const void* in;
+  int32_t in_w, in_h, in_type, bits;
+  void* out;
+  int32_t out_type;
+  int32_t target_w, target_h; /* display size; 0, 0 = the variant's 2x (3x for ravu-3x) */
+  int32_t precision;          /* MPX_PREC_* (| MPX_STORE_FP16)                          */
+  void* ready_stream;
+  int64_t deadline_us;
+} mpx_request;
+
+typedef struct mpx_queue mpx_queue;
+
+/* A request queue over `ctx` (which it shares: plans and weights stay resident under the
+ * context's HBM budget; the queue allocates no frame memory, outputs are the caller's).
+ * A scheduler thread admits requests in arrival order and groups them by plan: frames at
+ * their variant's default scale whose sizes fit max_w x max_h share launches whatever their
+ * sizes (mpx_run_frames_sized), the others group by size (mpx_run_frames); up to
+ * max_batch (<= MPX_MAX_FRAMES) frames per launch, launches spread over `streams` HIP
+ * streams of its own, at most 2 launches in flight per stream so that arrivals keep
+ * joining the next launch.  A device error fails every unfinished request and the queue
+ * (submits then return MPX_ERR_HIP).  Replaces mpv dispatching the hook's passes once per
+ * frame (e.g. ravu-lite-r3.hook:15-21, 119-126) with batched launches across requests. */
+int mpx_queue_create(mpx_ctx* ctx, int streams, int max_batch, int max_w, int max_h, mpx_queue** out);
+/* Finish every submitted request (run what is queued), then free the queue. */
+void mpx_queue_destroy(mpx_queue* q);
+/* Queue one request; *ticket identifies it.  Checks the variant, sizes, types and
+ * pointers first (nothing is queued on an error).  Never waits for the GPU. */
+int mpx_submit(mpx_queue* q, const mpx_request* r, uint64_t* ticket);
+/* MPX_OK: the request finished and its output is complete (visible to any stream);
+ * MPX_PENDING: not yet after waiting up to timeout_us (0: do not wait, < 0: wait until
+ * it finishes); a negative status: the request failed (message in mpx_last_error).
+ * latency_us (may be NULL): on MPX_OK or an error, the microseconds from mpx_submit to the
+ * moment the queue saw the request finish.  A ticket is retired once it returns MPX_OK or
+ * an error. */
+int mpx_poll(mpx_queue* q, uint64_t ticket, int64_t timeout_us, int64_t* latency_us);
+/* Launches issued so far (how many requests shared a launch is submitted / launches). */
+uint64_t mpx_queue_launches(const mpx_queue* q);
+/* hold != 0: the scheduler takes nothing from the queue (submits still queue); hold == 0:
+ * it takes everything queued again, grouped as usual -- to submit a set of frames that
+ * should share launches.  mpx_queue_destroy releases a hold. */
+int mpx_queue_hold(mpx_queue* q, int hold);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char* mpx_last_error(void);
 

@@ -26,7 +26,8 @@ CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
            "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_plan_set_shape", "mpx_resample",
-           "mpx_weights_verify", "mpx_run_static", "mpx_last_error")
+           "mpx_weights_verify", "mpx_run_static", "mpx_queue_create", "mpx_queue_destroy", "mpx_submit", "mpx_poll",
+           "mpx_queue_launches", "mpx_queue_hold", "mpx_last_error")
 MAX_FRAMES = 16
 STATIC_SCRATCH_BYTES = 64 << 20   # MPX_STATIC_SCRATCH_BYTES
 
@@ -50,6 +51,17 @@ class MpxIo(ctypes.Structure):
                 ("in_bstride", ctypes.c_int64), ("out", ctypes.c_void_p), ("out_pitch", ctypes.c_int64),
                 ("out_cstride", ctypes.c_int64), ("out_bstride", ctypes.c_int64), ("batch", ctypes.c_int32),
                 ("buckets", ctypes.c_void_p)]
+
+
+class MpxRequest(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("in_", ctypes.c_void_p), ("in_w", ctypes.c_int32), ("in_h", ctypes.c_int32),
+                ("in_type", ctypes.c_int32), ("bits", ctypes.c_int32), ("out", ctypes.c_void_p),
+                ("out_type", ctypes.c_int32), ("target_w", ctypes.c_int32), ("target_h", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("ready_stream", ctypes.c_void_p), ("deadline_us", ctypes.c_int64)]
+
+
+MPX_PENDING = 2
+STATUS[-9] = "MPX_ERR_DEADLINE"
 
 
 class MpxError(RuntimeError):
@@ -89,6 +101,12 @@ def _load():
                                ctypes.c_float, ctypes.c_float, vp]),
         "mpx_weights_verify": (i32, [ctypes.c_char_p, ctypes.c_char_p]),
         "mpx_last_error": (ctypes.c_char_p, []),
+        "mpx_queue_create": (i32, [vp, i32, i32, i32, i32, ctypes.POINTER(vp)]),
+        "mpx_queue_destroy": (None, [vp]),
+        "mpx_submit": (i32, [vp, ctypes.POINTER(MpxRequest), ctypes.POINTER(u64)]),
+        "mpx_poll": (i32, [vp, u64, i64, ctypes.POINTER(i64)]),
+        "mpx_queue_launches": (u64, [vp]),
+        "mpx_queue_hold": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -68,6 +68,15 @@ int main(int argc, char** argv) {
   if (mpx_plan_set_shape(NULL, 0, 0) != MPX_ERR_ARG || mpx_ctx_bytes(NULL) != 0) return 18;
   mpx_plan_destroy(NULL);
   mpx_destroy(NULL);
+  mpx_queue* q = NULL;
+  uint64_t ticket = 0;
+  mpx_request req;
+  memset(&req, 0, sizeof(req));
+  if (mpx_queue_create(NULL, 2, 4, 64, 64, &q) != MPX_ERR_ARG) return 21;
+  if (mpx_submit(NULL, &req, &ticket) != MPX_ERR_ARG || mpx_submit(q, NULL, &ticket) != MPX_ERR_ARG) return 22;
+  if (mpx_poll(NULL, 1, 0, NULL) != MPX_ERR_ARG || mpx_queue_launches(NULL) != 0) return 23;
+  if (mpx_queue_hold(NULL, 1) != MPX_ERR_ARG) return 24;
+  mpx_queue_destroy(NULL);
   /* no usable device in this process: a status, never an abort */
   mpx_ctx* ctx = NULL;
   int crc = mpx_create(0, 0, argv[1], &ctx);
