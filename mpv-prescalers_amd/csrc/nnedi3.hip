@@ -139,6 +139,59 @@ __device__ __forceinline__ void normalise(float* x, float& m, float& sd, float& 
   }
 }
 
+// Window statistics shared across the pixels of a staged tile.  A window is 8 lines of WH
+// samples -- AXIS 0: columns tx+a (a < 8) over rows ty..ty+WH-1; AXIS 1: rows ty+a over
+// columns tx..tx+WH-1 -- and neighbouring pixels share lines, so each line's mean and
+// centred sum of squares is computed once per staged tile (`build`), and a window combines
+// its 8 lines by the pairwise formula M2 = sum_a M2_a + WH * sum_a (mean_a - m)^2 (every term
+// non-negative: no cancellation in FP32), v = M2 / K, the same quantity as the two-pass
+// variance mean((x - m)^2).
+template <int WH, int AXIS, int TY>
+struct LineStats {
+  using G = Geo<WH, AXIS, TY>;
+  static constexpr int COLS = AXIS == 0 ? G::TW : NX;
+  static constexpr int N = (AXIS == 0 ? TY : G::TH) * COLS;
+  __device__ static void build(const float* tile, float2* tab) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      const int r = i / COLS, c = i - r * COLS;
+      float x[WH];
+#pragma unroll
+      for (int b = 0; b < WH; ++b) x[b] = AXIS == 0 ? tile[(r + b) * G::TW + c] : tile[r * G::TW + c + b];
+      float sum = x[0];
+#pragma unroll
+      for (int b = 1; b < WH; ++b) sum += x[b];
+      const float mu = sum * (1.0f / static_cast<float>(WH));
+      float m2 = 0.f;
+#pragma unroll
+      for (int b = 0; b < WH; ++b) {
+        const float d = x[b] - mu;
+        m2 = fmaf(d, d, m2);
+      }
+      tab[i] = make_float2(mu, m2);
+    }
+  }
+  // mean m and variance v (mean squared deviation) of pixel (tx, ty)'s window
+  __device__ static void window(const float2* tab, int tx, int ty, float& m, float& v) {
+    float2 e[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) e[a] = AXIS == 0 ? tab[ty * COLS + tx + a] : tab[(ty + a) * COLS + tx];
+    float sm = e[0].x, m2 = e[0].y;
+#pragma unroll
+    for (int a = 1; a < 8; ++a) {
+      sm += e[a].x;
+      m2 += e[a].y;
+    }
+    m = sm * 0.125f;
+    float dev = 0.f;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const float d = e[a].x - m;
+      dev = fmaf(d, d, dev);
+    }
+    v = fmaf(static_cast<float>(WH), dev, m2) * (1.0f / static_cast<float>(8 * WH));
+  }
+};
+
 template <int AXIS>
 __device__ __forceinline__ void store_result(const NPass& p, void* dst, int64_t pitch, int x, int y, float orig,
                                              float pred) {
@@ -279,9 +332,14 @@ struct Mma {
         w.lo[c][j] = (TERMS >= 3 && in) ? L[(static_cast<int64_t>(n) * 2 + mtx) * K + k] : zero;
       }
   }
-  // one pixel's normalised window (scaled by xscale) -> its LDS row(s); with FUSED_SCALE
-  // the window is centred only and is = 1/std
-  __device__ static void write_row(T* rh, T* rl, const float* xv, float xscale, bool scaled, float is) {
+  // one pixel's window (raw samples xv, mean m, is = 1/std) -> its LDS row(s): the
+  // normalised window x' = (x - m) * is (times xscale when scaled), split into 16-bit high
+  // and low parts.  FUSED_SCALE forms centre inside the conversion, x * is - m * is in one
+  // fused multiply-add: the rounding of m * is (<= m * is * 2^-24) reaches the output only
+  // multiplied by the window's std = 1 / is, so it adds < 1e-6 at most.
+  __device__ static void write_row(T* rh, T* rl, const float* xv, float m, float is, float xscale, bool scaled) {
+    const float c = -m * is;
+    const float sc = scaled ? is * xscale : is;
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       if constexpr (FUSED_SCALE) {
@@ -290,12 +348,17 @@ struct Mma {
           if (TERMS >= 2) rl[k] = O::cvt(0.0f);
           continue;
         }
-        const float x = k < K ? xv[k] : 0.f;
-        const T h = O::cvt(fmaf(x, is, 0.0f));
+        if (k >= K) {
+          rh[k] = O::cvt(0.0f);
+          if (TERMS >= 2) rl[k] = O::cvt(0.0f);
+          continue;
+        }
+        const float v = fmaf(xv[k], is, c);
+        const T h = O::cvt(v);
         rh[k] = h;
-        if (TERMS >= 2) rl[k] = O::cvt(fmaf(x, is, -O::back(h)));
+        if (TERMS >= 2) rl[k] = O::cvt(v - O::back(h));
       } else {
-        const float v = k < K ? (scaled ? xv[k] * xscale : xv[k]) : 0.f;
+        const float v = k < K ? (xv[k] - m) * sc : 0.f;
         const T h = O::cvt(v);
         rh[k] = h;
         if (TERMS >= 2) rl[k] = O::cvt(v - O::back(h));
@@ -353,13 +416,13 @@ struct Mma<F_FP8, K, TERMS> {
     w.sc = sc[n * 2 + mtx];
   }
   static constexpr bool FUSED_SCALE = false;
-  __device__ static void write_row(T* rh, T* /*rl*/, const float* xv, float /*xscale*/, bool /*scaled*/,
-                                   float /*is*/) {
-    constexpr float XSC = static_cast<float>(1 << kFp8XExp);
+  __device__ static void write_row(T* rh, T* /*rl*/, const float* xv, float m, float is, float /*xscale*/,
+                                   bool /*scaled*/) {
+    const float XSC = static_cast<float>(1 << kFp8XExp) * is;
     uint32_t hv[(K + 3) / 4], lv[(K + 3) / 4];
 #pragma unroll
     for (int k = 0; k < K; k += 2) {
-      const float v0 = xv[k] * XSC, v1 = xv[k + 1] * XSC;
+      const float v0 = (xv[k] - m) * XSC, v1 = (xv[k + 1] - m) * XSC;
       const uint32_t h = __builtin_amdgcn_cvt_pk_fp8_f32(v0, v1, 0, false);
       const float r0 = v0 - __builtin_amdgcn_cvt_f32_fp8(static_cast<int>(h), 0);
       const float r1 = v1 - __builtin_amdgcn_cvt_f32_fp8(static_cast<int>(h), 1);
@@ -425,7 +488,14 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   constexpr int NPX = NX * NY;            // 256 pixels per tile
   constexpr int NGROUP = NPX / 16;        // 16 pixel groups
   constexpr int PSL = 4 / NSL;            // pixel slices
+  using LS = LineStats<WH, AXIS, NYS>;
+  // the one-product FP16 forms on 8x6 windows take the window statistics from shared line
+  // statistics (their variance is good to a few ulp; the split forms, which resolve ~22 bits,
+  // keep the per-pixel two-pass variance, good to one).  On 8x4 windows the table's build
+  // pass and barrier cost more than the 32-term sums they save (measured 1-3% slower).
+  constexpr bool LINES = M::FUSED_SCALE && TERMS == 1 && WH == 6;
   __shared__ float tile[2][G::TH * G::TW];
+  __shared__ float2 stab[LINES ? LS::N : 1];   // line statistics of the staged tile
   __shared__ __attribute__((aligned(16))) T xhi[NPX * XS];
   __shared__ __attribute__((aligned(16))) T xlo[M::LO_ROWS ? NPX * XS : 16];
   __shared__ float part[NSL > 1 ? NSL : 1][NSL > 1 ? NPX : 1][2];
@@ -524,6 +594,11 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     ht.commit(tile[buf], p.src_dt, p.src_max, p.src_rmax);
     __syncthreads();
     if (L + static_cast<int>(gridDim.x) < total_tiles) issue(xcd_tile(L + gridDim.x, total_tiles));
+    if constexpr (LINES) {
+      // (the previous tile's reads of stab ended before the barrier above)
+      LS::build(tile[buf], stab);
+      __syncthreads();
+    }
 
     // the staged tile holds NYS = 2 * NY rows: two sub-tiles of NX x NY pixels, one
     // pixel per thread each, sharing one halo load and one block barrier
@@ -542,8 +617,18 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
           for (int b = 0; b < WH; ++b) xv[a * WH + b] = tl[G::at(tx, ty, a, b)];
         orig = tl[G::centre(tx, ty)];
         float is;
-        normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
-        M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED, is);
+        if constexpr (LINES) {
+          float v;
+          LS::window(stab, tx, ty, m, v);
+          // hardware reciprocal square root (within 1 ulp); the threshold test is exact
+          is = v >= kVarEps ? __builtin_amdgcn_rsqf(v) : 0.f;
+          sd = v * is;
+        } else {
+          // two-pass: the window comes back centred (FUSED_SCALE) or normalised
+          normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
+        }
+        M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, LINES ? m : 0.f,
+                     (LINES || M::FUSED_SCALE) ? is : 1.f, xscale, SCALED);
       }
 
       float wsum = 0.f, vsum = 0.f;
@@ -632,22 +717,16 @@ struct Mma32 {
         w.lo[c][j] = TERMS >= 3 ? L[i] : static_cast<T>(0.0f);
       }
   }
-  // one pixel's window -> its LDS row(s): x' * xscale (SCALED) or x', split into 16-bit
-  // high and low parts; with FUSED_SCALE the window is centred only and 1/std = is is
-  // applied inside the conversions
-  __device__ static void write_row(T* rh, T* rl, const float* xv, float xscale, bool scaled, float is) {
+  // one pixel's window (raw samples, mean m, is = 1/std) -> its LDS row(s), as Mma::write_row
+  __device__ static void write_row(T* rh, T* rl, const float* xv, float m, float is, float xscale, bool scaled) {
+    const float c = -m * is;
+    const float sc = scaled ? is * xscale : is;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      if constexpr (FUSED_SCALE) {
-        const T h = O::cvt(fmaf(xv[k], is, 0.0f));
-        rh[k] = h;
-        if (TERMS >= 2) rl[k] = O::cvt(fmaf(xv[k], is, -O::back(h)));
-      } else {
-        const float v = scaled ? xv[k] * xscale : xv[k];
-        const T h = O::cvt(v);
-        rh[k] = h;
-        if (TERMS >= 2) rl[k] = O::cvt(v - O::back(h));
-      }
+      const float v = FUSED_SCALE ? fmaf(xv[k], is, c) : (xv[k] - m) * sc;
+      const T h = O::cvt(v);
+      rh[k] = h;
+      if (TERMS >= 2) rl[k] = O::cvt(v - O::back(h));
     }
   }
   __device__ static void load_b(B& b, const T* xh, const T* xl, int px, int half) {
@@ -687,7 +766,14 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
   constexpr int NGROUP = NPX / 32;  // 8 pixel groups
   constexpr int PSL = 4 / NSL;      // pixel slices
   static_assert(NSL * PSL == 4, "4 waves per block");
+  using LS = LineStats<WH, AXIS, NYS>;
+  // the one-product FP16 forms on 8x6 windows take the window statistics from shared line
+  // statistics (their variance is good to a few ulp; the split forms, which resolve ~22 bits,
+  // keep the per-pixel two-pass variance, good to one).  On 8x4 windows the table's build
+  // pass and barrier cost more than the 32-term sums they save (measured 1-3% slower).
+  constexpr bool LINES = M::FUSED_SCALE && TERMS == 1 && WH == 6;
   __shared__ float tile[2][G::TH * G::TW];
+  __shared__ float2 stab[LINES ? LS::N : 1];   // line statistics of the staged tile
   __shared__ __attribute__((aligned(16))) T xhi[NPX * XS];
   __shared__ __attribute__((aligned(16))) T xlo[M::LO_ROWS ? NPX * XS : 8];
   __shared__ float part[NSL > 1 ? NSL : 1][NSL > 1 ? NPX : 1][2];
@@ -767,6 +853,11 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
     ht.commit(tile[buf], p.src_dt, p.src_max, p.src_rmax);
     __syncthreads();
     if (L + static_cast<int>(gridDim.x) < total_tiles) issue(xcd_tile(L + gridDim.x, total_tiles));
+    if constexpr (LINES) {
+      // (the previous tile's reads of stab ended before the barrier above)
+      LS::build(tile[buf], stab);
+      __syncthreads();
+    }
 
 #pragma unroll 1
     for (int sub = 0; sub < NYS / NY; ++sub) {
@@ -781,8 +872,18 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
           for (int b = 0; b < WH; ++b) xv[a * WH + b] = tl[G::at(tx, ty, a, b)];
         orig = tl[G::centre(tx, ty)];
         float is;
-        normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
-        M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, xscale, SCALED, is);
+        if constexpr (LINES) {
+          float v;
+          LS::window(stab, tx, ty, m, v);
+          // hardware reciprocal square root (within 1 ulp); the threshold test is exact
+          is = v >= kVarEps ? __builtin_amdgcn_rsqf(v) : 0.f;
+          sd = v * is;
+        } else {
+          // two-pass: the window comes back centred (FUSED_SCALE) or normalised
+          normalise<K, !M::FUSED_SCALE>(xv, m, sd, is);
+        }
+        M::write_row(xhi + threadIdx.x * XS, xlo + (M::LO_ROWS ? threadIdx.x * XS : 0), xv, LINES ? m : 0.f,
+                     (LINES || M::FUSED_SCALE) ? is : 1.f, xscale, SCALED);
       }
 
       float wsum = 0.f, vsum = 0.f;
