@@ -106,6 +106,64 @@ __device__ __attribute__((noinline)) int zoom_redecide(const float* t, int o) {
   return (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
 }
 
+// Stage the part of the PH x PW staging area that a tile's outputs read: the uh x uw
+// samples per plane around its base pixels (at 2x 8 x 20 of 12 x 36), clamp-to-edge, at
+// the area's row stride PW.  AR forms store each sample's anti-ringing powers beside it
+// ((0.1 + l)^32, (1.1 - l)^32 and the ^33 terms, once per sample instead of once per
+// output pixel that reads it).  Sample i of the dense P x uh x uw order is split with
+// multiply-shift quotients, exact for every i < P * PN (i * d < 2^20 for both divisors).
+template <int P, int PH, int PW, bool AR>
+__device__ __forceinline__ void stage_used(float* tile, float4* pw, const void* base, int dt, float in_max,
+                                           float in_rmax, int64_t pitch, int64_t cstride, int W, int H, int ox,
+                                           int oy, int uw, int uh) {
+  constexpr int PN = PH * PW, E = (P * PN + 255) / 256;
+  static_assert(P * PN * PN < (1 << 20) && PN * PW < (1 << 20), "multiply-shift quotients exact");
+  const int un = uw * uh, total = P * un;
+  const uint32_t mn = (1u << 20) / static_cast<uint32_t>(un) + 1u;
+  const uint32_t mw = (1u << 20) / static_cast<uint32_t>(uw) + 1u;
+  const int lsh = dt == DT_F32 ? 2 : (dt == DT_U8 ? 0 : 1);  // log2 element bytes
+  const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+  uint32_t v[E], sh[E];
+  int li[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = static_cast<int>(threadIdx.x) + 256 * e;
+    li[e] = -1;
+    if (i < total) {
+      const int c = P == 1 ? 0 : static_cast<int>((static_cast<uint32_t>(i) * mn) >> 20);
+      const int j = i - c * un;
+      const int r = static_cast<int>((static_cast<uint32_t>(j) * mw) >> 20);
+      const int col = j - r * uw;
+      const int gx = min(max(ox + col, 0), W - 1), gy = min(max(oy + r, 0), H - 1);
+      const uintptr_t a = b + (static_cast<uintptr_t>(c * cstride + static_cast<int64_t>(gy) * pitch + gx) << lsh);
+      v[e] = *reinterpret_cast<GlobalU32>(a & ~static_cast<uintptr_t>(3));
+      sh[e] = static_cast<uint32_t>(a & 3) * 8;
+      li[e] = c * PN + r * PW + col;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    if (li[e] >= 0) {
+      const uint32_t w = v[e] >> sh[e];
+      float l;
+      switch (dt) {
+        case DT_F32: l = __uint_as_float(v[e]); break;
+        case DT_F16: l = __half2float(__ushort_as_half(static_cast<unsigned short>(w & 0xffffu))); break;
+        case DT_U8: l = unorm_quotient(static_cast<float>(w & 0xffu), in_max, in_rmax); break;
+        default: l = unorm_quotient(static_cast<float>(w & 0xffffu), in_max, in_rmax); break;
+      }
+      tile[li[e]] = l;
+      if constexpr (AR) {
+        const float p0 = 0.1f + l, m0 = 1.1f - l;
+        float p32 = p0, m32 = m0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) { p32 *= p32; m32 *= m32; }
+        pw[li[e]] = make_float4(p32, m32, p32 * p0, m32 * m0);
+      }
+    }
+  }
+}
+
 // Pre-interpolated weights.  The phase of output column o repeats with period
 // out / gcd(in, out) (2 at 2x, 3 at 1.5x), so for such plans every (bucket row, phase pair)
 // is interpolated once, at plan creation, by the same lut_bilinear in the same operations:
@@ -186,11 +244,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
   const int nbx = phase[min(ox0 + ZT_X, out_w) - 1].x - bx0 + 1;
   const int nby = phy_tab[min(oy0 + ZT_Y, out_h) - 1].x - by0 + 1;
   const void* in = frame_in(io, f);
-  {
-    HaloTile<P, PH, PW, 256> h;
-    h.issue(in, io.in_dt, io.in_pitch, io.in_cstride, io.W, io.H, bx0 - (R - 1), by0 - (R - 1));
-    h.commit(tile, io.in_dt, io.in_max, io.in_rmax);
-  }
+  stage_used<P, PH, PW, AR>(tile, pw, in, io.in_dt, io.in_max, io.in_rmax, io.in_pitch, io.in_cstride, io.W, io.H,
+                            bx0 - (R - 1), by0 - (R - 1), nbx + N - 1, nby + N - 1);
   __syncthreads();
   // hash every base pixel the tile's outputs use (unclamped LUT row: bucket 9-row blocks
   // are clamped per texel by the bilinear fetch, as the sampler does)
@@ -210,16 +265,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
       const Bucket bk = hash_fast<RULE_LITE>(ta, tb, td, &near);
       bucket[by * ZB_X + bx] = near ? zoom_redecide<N, CH, PW, PN, TAB>(tile, off(0))
                                     : (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
-    }
-  }
-  if constexpr (AR) {
-    for (int i = threadIdx.x; i < P * PN; i += blockDim.x) {
-      const float l = tile[i];
-      const float p0 = 0.1f + l, m0 = 1.1f - l;
-      float p32 = p0, m32 = m0;
-#pragma unroll
-      for (int e = 0; e < 5; ++e) { p32 *= p32; m32 *= m32; }
-      pw[i] = make_float4(p32, m32, p32 * p0, m32 * m0);
     }
   }
   __syncthreads();

@@ -62,17 +62,23 @@ class Server:
     """Continuous-batching request loop over one device.
 
     max_batch: frames per launch (<= 16); streams: concurrent HIP streams;
-    hbm_budget: bytes for weights + plan scratch (the Context's budget) and, separately,
-    ``admit_bytes`` for the outputs of admitted, unfinished requests."""
+    hbm_budget: the loop's whole device-memory envelope, split in two: ``admit_bytes``
+    (default a quarter) for the outputs of admitted, unfinished requests, and the rest
+    for weights + plan scratch (the Context's budget, which evicts idle plans to stay
+    inside it).  Weights, scratch and outputs together never exceed hbm_budget."""
 
     def __init__(self, device=0, hbm_budget=16 << 30, admit_bytes=None, streams=4, max_batch=8,
                  plan_batch=None, mixed_sizes=(2560, 1440)):
         self.device = torch.device("cuda", device) if isinstance(device, int) else device
-        self.ctx = Context(self.device.index or 0, hbm_budget=hbm_budget)
+        self.admit_bytes = int(admit_bytes) if admit_bytes is not None else hbm_budget // 4
+        if not 0 < self.admit_bytes < hbm_budget:
+            raise ValueError("admit_bytes must be positive and below hbm_budget")
+        self.hbm_budget = int(hbm_budget)
+        self.ctx_budget = self.hbm_budget - self.admit_bytes
+        self.ctx = Context(self.device.index or 0, hbm_budget=self.ctx_budget)
         self.streams = [torch.cuda.Stream(self.device) for _ in range(streams)]
         self.max_batch = max(1, min(16, max_batch))
         self.plan_batch = plan_batch or self.max_batch
-        self.admit_bytes = admit_bytes if admit_bytes is not None else hbm_budget // 2
         # frames up to this size (width, height) with the variant's default display size
         # share launches across sizes (one plan of this capacity per variant); None: one
         # plan and launch group per size
@@ -185,7 +191,8 @@ class Server:
 
     def _admit_locked(self):
         """Admit queued requests in arrival order while the outputs in flight fit
-        admit_bytes; requests whose plan cannot be made fail here."""
+        admit_bytes; requests whose plan cannot be made, or whose output alone exceeds
+        admit_bytes, fail here."""
         admitted = []
         while self._queue:
             r = self._queue[0]
@@ -202,7 +209,13 @@ class Server:
                 r.key = key
                 shape = plan.frame_out_shape(r.frame) if key[-1] else plan.out_shape(r.frame)
                 r.bytes = int(torch.Size(shape).numel()) * torch.finfo(r.out_dtype).bits // 8
-            if self._inflight + r.bytes > self.admit_bytes and self._inflight > 0:
+                if r.bytes > self.admit_bytes:
+                    self._queue.popleft()
+                    r.error = ValueError("output of %d bytes exceeds admit_bytes (%d)" % (r.bytes, self.admit_bytes))
+                    r.t_done = time.perf_counter()
+                    r._done.set()
+                    continue
+            if self._inflight + r.bytes > self.admit_bytes:
                 break
             self._queue.popleft()
             self._inflight += r.bytes

@@ -61,18 +61,28 @@ def test_mix_results_equal_alone():
 
 
 def test_admission_under_small_budget():
-    """Outputs of in-flight requests may not exceed admit_bytes: requests queue and all
-    complete."""
+    """One envelope: the context gets hbm_budget - admit_bytes, outputs of in-flight
+    requests may not exceed admit_bytes (requests queue and all complete), and a request
+    whose output alone exceeds admit_bytes fails without stopping the loop."""
     from mpx.serving import Server
     one = 2 * 2 * 160 * 2 * 90 * 2  # one 320x180 fp16 output ... twice
     srv = Server(device=0, hbm_budget=2 << 30, admit_bytes=one, streams=2, max_batch=8)
     try:
+        assert srv.ctx_budget + srv.admit_bytes == 2 << 30
         rs = [srv.submit("ravu-lite-r3", _frame(160, 90, 1, s)) for s in range(12)]
+        big = srv.submit("ravu-lite-r3", _frame(320, 180, 1, 99))      # 4x one output
+        after = srv.submit("ravu-lite-r3", _frame(160, 90, 1, 100))
         for r in rs:
             r.wait(timeout=60)
+        with pytest.raises(ValueError):
+            big.wait(timeout=60)
+        assert after.wait(timeout=60).shape == (180, 320)
         assert srv._inflight == 0
+        assert 0 < srv.ctx.bytes() <= srv.ctx_budget
     finally:
         srv.close()
+    with pytest.raises(ValueError):
+        Server(device=0, hbm_budget=1 << 20, admit_bytes=1 << 20)
 
 
 def test_bad_request_fails_alone():
@@ -96,7 +106,9 @@ def test_plan_eviction_under_context_budget():
     import mpx
     sizes = [(256, 256), (200, 200), (160, 160), (256, 256)]
     budget = 64 * 1024 + 256 * 256 * 4 + 32 * 1024   # weights + the largest int11 scratch + slack
-    srv = Server(device=0, hbm_budget=budget, admit_bytes=1 << 30, streams=1, max_batch=1, mixed_sizes=None)
+    srv = Server(device=0, hbm_budget=budget + (1 << 30), admit_bytes=1 << 30, streams=1, max_batch=1,
+                 mixed_sizes=None)
+    assert srv.ctx_budget == budget
     try:
         frames = [_frame(w, h, 1, 30 + i) for i, (w, h) in enumerate(sizes)]
         rs = [srv.submit("ravu-r2", f) for f in frames]
