@@ -439,6 +439,7 @@ def kernel_only_ravu(ctx, steps, torch):
         xin = torch.from_numpy(fr.reshape((b, planes, ih, iw) if planes == 3 else (b, ih, iw))).to(dt).to("cuda:0")
         pl = ctx.plan(name, iw, ih, target=tgt, in_dtype=dt, out_dtype=dt, max_batch=b)
         out = pl.run(xin)
+        pl.tune(xin, out, iters=3)   # the tile shape the bench lines use
         for _ in range(steps):
             pl.run(xin, out=out)
     torch.cuda.synchronize()
@@ -531,6 +532,8 @@ def main():
             fr = np.stack([make_plane(iw, ih, seed=i) for i in range(b * planes)]).astype(np.float32)
             xin = torch.from_numpy(fr.reshape((b, planes, ih, iw) if planes == 3 else (b, ih, iw))).to(dt).to(dev)
             pl = ctx.plan(name, iw, ih, target=tgt, in_dtype=dt, out_dtype=dt, max_batch=b)
+            pl.tune(xin, iters=3)                        # tile shape, kept by the plan
+            tile = pl.tile()
             t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
             opx = pl.info["out_w"] * pl.info["out_h"]
             io_bytes = (iw * ih + opx) * planes * (4 if dt == torch.float32 else 2)
@@ -539,7 +542,8 @@ def main():
             secondary[key] = {
                 "mpx_s": round(b * 10 * opx / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
                 "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
-                "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4)}
+                "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4),
+                "tile": [tile["w"], tile["h"]]}
             lds = lds_bytes_per_frame(key)
             if lds:
                 # LDS traffic per frame from the hardware counters of the same workload
@@ -555,11 +559,14 @@ def main():
                 xin = torch.from_numpy(fr.reshape(b, 3, ih, iw)).to(dev)
                 pl = zctx.plan("ravu-zoom-ar-r3-rgb", iw, ih, target=(tw, th), in_dtype=torch.float16,
                                out_dtype=torch.float16, max_batch=b)
+                pl.tune(xin, iters=3)
+                tile = pl.tile()
                 t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
                 io_bytes = (iw * ih + tw * th) * 3 * 2
                 rec = {"mpx_s": round(b * 10 * tw * th / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
                        "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
                        "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4),
+                       "tile": [tile["w"], tile["h"]],
                        "weights": "synthetic AR table (|r3 main table|; the reference ships none)"}
                 err = parity_record("ravu-zoom-ar-r3-rgb %dx%d->%dx%d" % (iw, ih, tw, th))
                 if err:

@@ -162,10 +162,12 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
  * (warm-up + `iters` timed runs per candidate, HIP events on hip_stream, synchronising)
  * over the candidate shapes and keeps the fastest in the plan.  NNEDI3 MFMA plans: the
  * persistent grid (blocks per CU) and, for nns = 64 and 128, the neuron split across waves.
- * Other plans have no tunable shape: returns MPX_OK with *grid_per_cu = 0.  Writes the
- * output buffers.  Outputs (may be NULL): the chosen blocks per CU (0 = occupancy
- * maximum) and split, and the best time per run in microseconds.  Replaces the fixed
- * workgroup shape of the compute shaders (e.g. compute/ravu-3x-r3.hook:22 `//!COMPUTE`). */
+ * RAVU plans (lite, 3x, full, zoom): the tile shape of their kernels (read the choice back
+ * with mpx_plan_tile); *grid_per_cu and *split are 0.  NNEDI3 FP32 plans have no tunable
+ * shape: MPX_OK, nothing run.  Writes the output buffers.  Outputs (may be NULL): the
+ * chosen blocks per CU (0 = occupancy maximum) and split, and the best time per run in
+ * microseconds.  Replaces the fixed workgroup shape of the compute shaders (e.g.
+ * compute/ravu-3x-r3.hook:22 `//!COMPUTE 96 24 32 8`, compute/ravu-zoom-r3.hook:23). */
 int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters, int* grid_per_cu, int* split,
                   float* best_us);
 
@@ -176,6 +178,18 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
  * accept only (0, 0).  Enqueues nothing.  No counterpart in the reference (its compute
  * shaders fix the workgroup shape, e.g. compute/ravu-3x-r3.hook:22). */
 int mpx_plan_set_shape(mpx_plan* plan, int grid_per_cu, int split);
+
+/* Tile shape of a RAVU plan's kernels: candidate `tile` of the family's list (0 = the
+ * default; mpx_plan_tune picks the fastest).  One block of the plan's kernels covers a
+ * tile_w x tile_h block of input pixels (ravu-lite, ravu-3x, full RAVU) or output pixels
+ * (ravu-zoom), one thread per pixel.  mpx_plan_tile outputs (may be NULL): the current
+ * index, the number of candidates (1 for NNEDI3 plans, with width = height = 0) and the
+ * current tile's size.  mpx_plan_set_tile enqueues nothing and changes no result, only
+ * speed; call it while no other thread runs the plan.  No counterpart in the reference
+ * (the compute shaders fix their workgroup, e.g. compute/ravu-lite-r3.hook:22
+ * `//!COMPUTE 64 16 32 8`, compute/ravu-zoom-r3.hook:23 `//!COMPUTE 32 8`). */
+int mpx_plan_set_tile(mpx_plan* plan, int tile);
+int mpx_plan_tile(const mpx_plan* plan, int* tile, int* count, int* tile_w, int* tile_h);
 
 /* ---- the scaling pass that consumes a prescaler's OFFSET ---------------------------- */
 #define MPX_FILTER_BILINEAR 0

@@ -549,18 +549,18 @@ static int launch_all(mpx_plan* plan, const mpx::Io& d, hipStream_t st, void* in
     const Variant& v = *plan->v;
     switch (v.family) {
       case MPX_FAMILY_RAVU_LITE:
-        e = mpx::launch_ravu_lite(v.radius, v.ar != 0, d, plan->lut, st);
+        e = mpx::launch_ravu_lite(v.radius, v.ar != 0, d, plan->lut, plan->tile, st);
         break;
       case MPX_FAMILY_RAVU_3X:
-        e = mpx::launch_ravu_3x(v.radius, v.channels, d, plan->lut, st);
+        e = mpx::launch_ravu_3x(v.radius, v.channels, d, plan->lut, plan->tile, st);
         break;
       case MPX_FAMILY_RAVU:
         e = mpx::launch_ravu(v.radius, v.channels, d, plan->lut, static_cast<float*>(interm), plan->scratch_frame,
-                             st);
+                             plan->tile, st);
         break;
       case MPX_FAMILY_RAVU_ZOOM:
         e = mpx::launch_ravu_zoom(v.radius, v.ar != 0, v.channels, d, plan->lut, plan->lut_ar, plan->zoom,
-                                  inf.out_w, inf.out_h, st);
+                                  inf.out_w, inf.out_h, plan->tile, st);
         break;
       case MPX_FAMILY_NNEDI3:
         e = mpx::launch_nnedi3(plan->nw, d, inf.scale_x == 2, inf.scale_y == 2, static_cast<float*>(interm),
@@ -650,6 +650,45 @@ int mpx_plan_set_shape(mpx_plan* plan, int grid_per_cu, int split) {
   return MPX_OK;
 }
 
+// Tile shapes of the plan's family (1 for families without a choice) and the shape of
+// candidate t in pixels.
+static int tile_count(const mpx_plan* plan) {
+  switch (plan->v->family) {
+    case MPX_FAMILY_RAVU_LITE: return mpx::kLiteTileCount;
+    case MPX_FAMILY_RAVU_3X: return mpx::k3xTileCount;
+    case MPX_FAMILY_RAVU: return mpx::kRavuTileCount;
+    case MPX_FAMILY_RAVU_ZOOM: return mpx::kZoomTileCount;
+    default: return 1;
+  }
+}
+static mpx::TileShape tile_shape(const mpx_plan* plan, int t) {
+  switch (plan->v->family) {
+    case MPX_FAMILY_RAVU_LITE: return mpx::kLiteTiles[t];
+    case MPX_FAMILY_RAVU_3X: return mpx::k3xTiles[t];
+    case MPX_FAMILY_RAVU: return mpx::kRavuTiles[t];
+    case MPX_FAMILY_RAVU_ZOOM: return mpx::kZoomTiles[t];
+    default: return mpx::TileShape{0, 0};
+  }
+}
+
+int mpx_plan_set_tile(mpx_plan* plan, int tile) {
+  if (!plan) return fail(MPX_ERR_ARG, "plan is NULL");
+  if (tile < 0 || tile >= tile_count(plan))
+    return fail(MPX_ERR_ARG, "tile outside 0.." + std::to_string(tile_count(plan) - 1) + " for this plan");
+  plan->tile = tile;
+  return MPX_OK;
+}
+
+int mpx_plan_tile(const mpx_plan* plan, int* tile, int* count, int* tile_w, int* tile_h) {
+  if (!plan) return fail(MPX_ERR_ARG, "plan is NULL");
+  const mpx::TileShape sh = tile_shape(plan, plan->tile);
+  if (tile) *tile = plan->tile;
+  if (count) *count = tile_count(plan);
+  if (tile_w) *tile_w = sh.w;
+  if (tile_h) *tile_h = sh.h;
+  return MPX_OK;
+}
+
 int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters, int* grid_per_cu, int* split,
                   float* best_us) {
   if (!plan || !io) return fail(MPX_ERR_ARG, "plan/io is NULL");
@@ -658,7 +697,21 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
   if (split) *split = 0;
   if (best_us) *best_us = 0.f;
   const bool nn = plan->v->family == MPX_FAMILY_NNEDI3 && plan->precision != MPX_PREC_FP32;
-  if (!nn || !plan->info.applies) return MPX_OK;
+  const int ntile = tile_count(plan);
+  if ((!nn && ntile == 1) || !plan->info.applies) return MPX_OK;
+  // candidates: NNEDI3 MFMA plans sweep the persistent grid and the neuron split, RAVU
+  // plans their family's tile shapes
+  struct Shape {
+    int grid, split, tile;
+  };
+  std::vector<Shape> cand;
+  if (nn) {
+    const int nsplit = plan->v->nns == 64 || plan->v->nns == 128 ? 2 : 1;
+    for (int sp = 0; sp < nsplit; ++sp)
+      for (int g : {0, 1, 2, 3, 4, 6, 8}) cand.push_back({g, sp, 0});
+  } else {
+    for (int t = 0; t < ntile; ++t) cand.push_back({0, 0, t});
+  }
   mpx_host::DeviceScope dscope;
   if (int drc = dscope.enter(plan->ctx->device)) return drc;
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
@@ -668,45 +721,50 @@ int mpx_plan_tune(mpx_plan* plan, const mpx_io* io, void* hip_stream, int iters,
     (void)hipEventDestroy(e0);
     return fail(MPX_ERR_HIP, "hipEventCreate failed");
   }
-  const int grids[] = {0, 1, 2, 3, 4, 6, 8};
-  const int nsplit = nn && (plan->v->nns == 64 || plan->v->nns == 128) ? 2 : 1;
   const mpx::NnediWeights saved = plan->nw;
-  auto set_shape = [&](int g, int sp) {
-    plan->nw.grid_per_cu = g;
-    plan->nw.split = sp;
+  const int saved_tile = plan->tile;
+  auto set_shape = [&](const Shape& c) {
+    if (nn) {
+      plan->nw.grid_per_cu = c.grid;
+      plan->nw.split = c.split;
+    }
+    plan->tile = c.tile;
   };
   float best = 0.f;
-  int best_g = 0, best_s = 0, rc = MPX_OK;
-  for (int sp = 0; sp < nsplit && rc == MPX_OK; ++sp)
-    for (int g : grids) {
-      set_shape(g, sp);
-      rc = mpx_run_ex(plan, io, hip_stream);  // warm-up; also validates io
-      if (rc != MPX_OK) break;
-      if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(MPX_ERR_HIP, "hipEventRecord failed"); break; }
-      for (int i = 0; i < iters && rc == MPX_OK; ++i) rc = mpx_run_ex(plan, io, hip_stream);
-      if (rc != MPX_OK) break;
-      float ms = 0.f;
-      if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = fail(MPX_ERR_HIP, "event timing failed");
-        break;
-      }
-      const float us = ms * 1000.f / static_cast<float>(iters);
-      if (best == 0.f || us < best) {
-        best = us;
-        best_g = g;
-        best_s = sp;
-      }
+  Shape best_c{0, 0, 0};
+  int rc = MPX_OK;
+  for (const Shape& c : cand) {
+    set_shape(c);
+    rc = mpx_run_ex(plan, io, hip_stream);  // warm-up; also validates io
+    if (rc != MPX_OK) break;
+    if (hipEventRecord(e0, st) != hipSuccess) {
+      rc = fail(MPX_ERR_HIP, "hipEventRecord failed");
+      break;
     }
+    for (int i = 0; i < iters && rc == MPX_OK; ++i) rc = mpx_run_ex(plan, io, hip_stream);
+    if (rc != MPX_OK) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(MPX_ERR_HIP, "event timing failed");
+      break;
+    }
+    const float us = ms * 1000.f / static_cast<float>(iters);
+    if (best == 0.f || us < best) {
+      best = us;
+      best_c = c;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != MPX_OK) {
     plan->nw = saved;
+    plan->tile = saved_tile;
     return rc;
   }
-  set_shape(best_g, best_s);
-  if (grid_per_cu) *grid_per_cu = best_g;
-  if (split) *split = best_s;
+  set_shape(best_c);
+  if (grid_per_cu) *grid_per_cu = best_c.grid;
+  if (split) *split = best_c.split;
   if (best_us) *best_us = best;
   return MPX_OK;
 }

@@ -181,6 +181,7 @@ struct mpx_plan {
   const __half* lut_ar = nullptr;     // zoom-ar LUT
   mpx::NnediWeights nw{};             // NNEDI3 packed weights (and their launch shape)
   mpx::ZoomPlan zoom{};               // zoom phase tables
+  int tile = 0;                       // RAVU families: index into the family's tile shapes
   void* scratch = nullptr;            // intermediates, max_batch frames
   size_t scratch_bytes = 0;
   int64_t scratch_frame = 0;          // elements per frame

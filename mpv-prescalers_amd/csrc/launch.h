@@ -25,11 +25,28 @@ inline int device_cu_count() {
   return n;
 }
 
+// Tile shapes each RAVU family's kernels are built for: one block per tile, one thread per
+// pixel of it (input pixels for lite / 3x / full RAVU, output pixels for zoom).  A plan
+// launches candidate `tile` of its family's list (0 = the default, the best on average in
+// A/B runs over frame sizes and channel forms; the others win for some sizes and forms by
+// 2-6 %), chosen by mpx_plan_tune or set with mpx_plan_set_tile.
+struct TileShape {
+  int w, h;
+};
+constexpr TileShape kLiteTiles[] = {{64, 4}, {32, 8}, {32, 4}};
+constexpr TileShape k3xTiles[] = {{32, 8}, {64, 4}};
+constexpr TileShape kRavuTiles[] = {{32, 16}, {64, 8}, {32, 8}};
+constexpr TileShape kZoomTiles[] = {{32, 8}, {16, 16}};
+constexpr int kLiteTileCount = sizeof(kLiteTiles) / sizeof(kLiteTiles[0]);
+constexpr int k3xTileCount = sizeof(k3xTiles) / sizeof(k3xTiles[0]);
+constexpr int kRavuTileCount = sizeof(kRavuTiles) / sizeof(kRavuTiles[0]);
+constexpr int kZoomTileCount = sizeof(kZoomTiles) / sizeof(kZoomTiles[0]);
+
 // Plane copy with element-type conversion (the output of a hook whose WHEN is false).
 hipError_t launch_copy(const Io& io, int planes, hipStream_t st);
 
-// RAVU-lite(-ar), r in {2,3,4}; lut = FP16 [288][ceil(n^2/2)][4].
-hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, hipStream_t st);
+// RAVU-lite(-ar), r in {2,3,4}; lut = FP16 [288][ceil(n^2/2)][4]; tile: index into kLiteTiles.
+hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, int tile, hipStream_t st);
 
 // Separable resampler (filter MPX_FILTER_*), io.W x io.H -> out_w x out_h, with offsets.
 hipError_t launch_resample(const Io& io, int out_w, int out_h, int planes, int filter, float off_x, float off_y,

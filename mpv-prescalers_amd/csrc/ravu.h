@@ -30,16 +30,17 @@ struct ZoomPlan {
 int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w, int out_h);
 void zoom_plan_destroy(mpx_ctx* ctx, mpx_plan* plan);
 
-// RAVU-3x, r in {2,3,4}; channels MPX_CH_*; lut = FP16 [216][2*ceil(n^2/2)][4].
-hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, hipStream_t st);
+// RAVU-3x, r in {2,3,4}; channels MPX_CH_*; lut = FP16 [216][2*ceil(n^2/2)][4]; tile: index
+// into k3xTiles.
+hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, int tile, hipStream_t st);
 
 // RAVU full: int11 into `int11` scratch (frame stride int11_frame floats), then int10/int01
-// and the interleaved 2x output.  lut = FP16 [648][n^2/8][4].
+// and the interleaved 2x output.  lut = FP16 [648][n^2/8][4]; tile: index into kRavuTiles.
 hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
-                       hipStream_t st);
+                       int tile, hipStream_t st);
 
-// RAVU-zoom(-ar) to out_w x out_h.
+// RAVU-zoom(-ar) to out_w x out_h; tile: index into kZoomTiles (output pixels per block).
 hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
-                            const ZoomPlan& zp, int out_w, int out_h, hipStream_t st);
+                            const ZoomPlan& zp, int out_w, int out_h, int tile, hipStream_t st);
 
 }  // namespace mpx

@@ -18,18 +18,15 @@
 
 namespace mpx {
 
-constexpr int RT_X = 64;
-constexpr int RT_Y = 4;
-// Full RAVU: 64 x 8 tiles on 512-thread blocks (one site per thread).  Their halo is up to
-// 9 rows, so the taller tile stages about a third fewer samples per site than 64 x 4 would
-// (r4 lattice: 73 x 17 instead of 73 x 13 per 2x as many sites).
-constexpr int RF_Y = 8;
-constexpr int RF_T = RT_X * RF_Y;
+// Tiles: TX x TY pixels per block of TX * TY threads, one input pixel (3x) or site (full
+// RAVU) per thread, from the families' shape lists (k3xTiles, kRavuTiles).  Full RAVU's halo
+// is up to 9 rows, so its default tile is 32 x 16 on 512 threads: r4 lattice 41 x 25
+// staged samples per 512 sites, against 73 x 13 per 256 for a 64 x 4 tile.
 
 
 // Stage P planes of a frame into LDS: tile[c][r][col] = plane c at
 // (clamp(ox + col), clamp(oy + r)), PH x PW per plane; all loads in flight at once.
-template <int P, int PH, int PW, int NT = 256>
+template <int P, int PH, int PW, int NT>
 __device__ __forceinline__ void stage_planes(float* tile, const void* base, int dt, float in_max, float in_rmax,
                                              int64_t pitch, int64_t cstride, int W, int H, int ox, int oy) {
   HaloTile<P, PH, PW, NT> h;
@@ -95,26 +92,26 @@ __device__ __forceinline__ Bucket hash_site(const float* t, int o) {
 
 // ----------------------------------------------------------------------------- RAVU-3x
 
-template <int R, int CH>
-__global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __restrict__ lut, int total) {
+template <int R, int CH, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void ravu_3x_kernel(Io io, const __half* __restrict__ lut, int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R - 1;
   constexpr int HALO = R - 1;
-  constexpr int PW = RT_X + 2 * HALO, PH = RT_Y + 2 * HALO, PN = PW * PH;
+  constexpr int PW = TX + 2 * HALO, PH = TY + 2 * HALO, PN = PW * PH;
   constexpr int C = (N * N - 1) / 2;
   constexpr int NTEX = 2 * (C + 1);
   __shared__ float tile[staged_planes<CH>() * PN];
-  const FrameTile ft = frame_tile<RT_X, RT_Y>(io, blockIdx.x, total);
+  const FrameTile ft = frame_tile<TX, TY>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  stage_planes<P, PH, PW>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - HALO, y0 - HALO);
+  stage_planes<P, PH, PW, TX * TY>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - HALO, y0 - HALO);
   __syncthreads();
   if constexpr (CH == 2) {
     stage_luma<CH>(tile, PN);
     __syncthreads();
   }
-  const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
   auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };  // sample i = X*N + Y
@@ -190,18 +187,27 @@ __global__ __launch_bounds__(256) void ravu_3x_kernel(Io io, const __half* __res
   }
 }
 
-hipError_t launch_ravu_3x(int r, int channels, const Io& io_, const __half* lut, hipStream_t st) {
-  Io io = io_;
-  const int total = launch_tiles<RT_X, RT_Y>(io);
-  dim3 grid(total);
+template <int T>
+static hipError_t launch_3x_tile(int r, int channels, Io io, const __half* lut, hipStream_t st) {
+  constexpr int TX = k3xTiles[T].w, TY = k3xTiles[T].h;
+  const int total = launch_tiles<TX, TY>(io);
 #define MPX_3X(R, CH)                                                                                  \
   if (r == R && channels == CH) {                                                                      \
-    hipLaunchKernelGGL((ravu_3x_kernel<R, CH>), grid, dim3(256), 0, st, io, lut, total);               \
+    hipLaunchKernelGGL((ravu_3x_kernel<R, CH, TX, TY>), dim3(total), dim3(TX * TY), 0, st, io, lut, total); \
     return hipGetLastError();                                                                          \
   }
   MPX_3X(2, 0) MPX_3X(2, 1) MPX_3X(2, 2) MPX_3X(3, 0) MPX_3X(3, 1) MPX_3X(3, 2) MPX_3X(4, 0) MPX_3X(4, 1) MPX_3X(4, 2)
 #undef MPX_3X
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, int tile, hipStream_t st) {
+  static_assert(k3xTileCount == 2, "one case per tile shape");
+  switch (tile) {
+    case 0: return launch_3x_tile<0>(r, channels, io, lut, st);
+    case 1: return launch_3x_tile<1>(r, channels, io, lut, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // ----------------------------------------------------------------------------- RAVU full
@@ -245,25 +251,25 @@ __device__ __forceinline__ void ravu_taps(const __half* lrow, S s, float* res) {
 }
 
 // step1: int11 from the N = 2R window at offsets -(R-1)..R (ravu-r3.hook:22-146).
-template <int R, int CH>
-__global__ __launch_bounds__(RF_T) __attribute__((amdgpu_waves_per_eu(6))) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
+template <int R, int CH, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(6))) void ravu_step1_kernel(Io io, const __half* __restrict__ lut, float* int11,
                                                          int64_t int11_frame, int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
-  constexpr int PW = RT_X + N - 1, PH = RF_Y + N - 1, PN = PW * PH;
+  constexpr int PW = TX + N - 1, PH = TY + N - 1, PN = PW * PH;
   __shared__ float tile[staged_planes<CH>() * PN];
-  const FrameTile ft = frame_tile<RT_X, RF_Y>(io, blockIdx.x, total);
+  const FrameTile ft = frame_tile<TX, TY>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
-  stage_planes<P, PH, PW, RF_T>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
+  stage_planes<P, PH, PW, TX * TY>(tile, in, io.in_dt, io.in_max, io.in_rmax, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - (R - 1),
                   y0 - (R - 1));
   __syncthreads();
   if constexpr (CH == 2) {
     stage_luma<CH>(tile, PN);
     __syncthreads();
   }
-  const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
   auto off = [&](int i) { return (ty + i % N) * PW + tx + i / N; };
@@ -291,22 +297,22 @@ __device__ __host__ constexpr int lat2y(int X, int Y) { return STEP == 2 ? Y - X
 // for one output row: STEP 2 writes row 2y = (input, int10) pairs, STEP 3 writes row
 // 2y+1 = (int01, int11) pairs (ravu-r3.hook:426-437), so both kernels store contiguous
 // pairs and each holds one prediction's registers only.
-template <int R, int CH, int STEP>
-__global__ __launch_bounds__(RF_T) void ravu_lattice_kernel(Io io, const __half* __restrict__ lut,
+template <int R, int CH, int STEP, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void ravu_lattice_kernel(Io io, const __half* __restrict__ lut,
                                                            const float* __restrict__ int11, int64_t int11_frame,
                                                            int total) {
   constexpr int P = CH == 0 ? 1 : 3;
   constexpr int N = 2 * R;
-  constexpr int PW = RT_X + 2 * R + 1, PH = RF_Y + 2 * R + 1, PN = PW * PH;  // offsets -R..R
+  constexpr int PW = TX + 2 * R + 1, PH = TY + 2 * R + 1, PN = PW * PH;  // offsets -R..R
   __shared__ float hk[staged_planes<CH>() * PN];   // input (HOOKED)
   __shared__ float i11[staged_planes<CH>() * PN];  // ravu_int11
-  const FrameTile ft = frame_tile<RT_X, RF_Y>(io, blockIdx.x, total);
+  const FrameTile ft = frame_tile<TX, TY>(io, blockIdx.x, total);
   const int f = ft.f;
   const int x0 = ft.x0, y0 = ft.y0;
   const void* in = frame_in(io, f);
   const int64_t plane = static_cast<int64_t>(ft.W) * ft.H;
   {
-    HaloTile<P, PH, PW, RF_T> a, b;  // both tiles' loads in flight together
+    HaloTile<P, PH, PW, TX * TY> a, b;  // both tiles' loads in flight together
     a.issue(in, io.in_dt, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - R, y0 - R);
     b.issue(int11 + f * int11_frame, DT_F32, ft.W, plane, ft.W, ft.H, x0 - R, y0 - R);
     a.commit(hk, io.in_dt, io.in_max, io.in_rmax);
@@ -318,7 +324,7 @@ __global__ __launch_bounds__(RF_T) void ravu_lattice_kernel(Io io, const __half*
     stage_luma<CH>(i11, PN);
     __syncthreads();
   }
-  const int tx = threadIdx.x % RT_X, ty = threadIdx.x / RT_X;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
   const int base = (ty + R) * PW + tx + R;  // LDS offset of (x, y)
@@ -346,26 +352,40 @@ __global__ __launch_bounds__(RF_T) void ravu_lattice_kernel(Io io, const __half*
   }
 }
 
-hipError_t launch_ravu(int r, int channels, const Io& io_, const __half* lut, float* int11, int64_t int11_frame,
-                       hipStream_t st) {
-  Io io = io_;
-  const int total = launch_tiles<RT_X, RF_Y>(io);
-  dim3 grid(total);
-#define MPX_RAVU(R, CH)                                                                                        \
-  if (r == R && channels == CH) {                                                                              \
-    hipLaunchKernelGGL((ravu_step1_kernel<R, CH>), grid, dim3(RF_T), 0, st, io, lut, int11, int11_frame, total);       \
-    hipError_t e = hipGetLastError();                                                                          \
-    if (e != hipSuccess) return e;                                                                             \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2>), grid, dim3(RF_T), 0, st, io, lut, int11, int11_frame, total);  \
-    e = hipGetLastError();                                                                                     \
-    if (e != hipSuccess) return e;                                                                             \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3>), grid, dim3(RF_T), 0, st, io, lut, int11, int11_frame, total);  \
-    return hipGetLastError();                                                                                  \
+template <int T>
+static hipError_t launch_ravu_tile(int r, int channels, Io io, const __half* lut, float* int11, int64_t int11_frame,
+                                   hipStream_t st) {
+  constexpr int TX = kRavuTiles[T].w, TY = kRavuTiles[T].h;
+  const int total = launch_tiles<TX, TY>(io);
+  const dim3 grid(total), block(TX * TY);
+#define MPX_RAVU(R, CH)                                                                                         \
+  if (r == R && channels == CH) {                                                                               \
+    hipLaunchKernelGGL((ravu_step1_kernel<R, CH, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame, total); \
+    hipError_t e = hipGetLastError();                                                                           \
+    if (e != hipSuccess) return e;                                                                              \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame,    \
+                       total);                                                                                  \
+    e = hipGetLastError();                                                                                      \
+    if (e != hipSuccess) return e;                                                                              \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame,    \
+                       total);                                                                                  \
+    return hipGetLastError();                                                                                   \
   }
   MPX_RAVU(2, 0) MPX_RAVU(2, 1) MPX_RAVU(2, 2) MPX_RAVU(3, 0) MPX_RAVU(3, 1) MPX_RAVU(3, 2) MPX_RAVU(4, 0)
   MPX_RAVU(4, 1) MPX_RAVU(4, 2)
 #undef MPX_RAVU
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
+                       int tile, hipStream_t st) {
+  static_assert(kRavuTileCount == 3, "one case per tile shape");
+  switch (tile) {
+    case 0: return launch_ravu_tile<0>(r, channels, io, lut, int11, int11_frame, st);
+    case 1: return launch_ravu_tile<1>(r, channels, io, lut, int11, int11_frame, st);
+    case 2: return launch_ravu_tile<2>(r, channels, io, lut, int11, int11_frame, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace mpx

@@ -4,9 +4,9 @@
 // ravu-lite-ar-r3.hook:90-182 (anti-ringing); the compute flavour
 // compute/ravu-lite-r3.hook:15-105 is the same math with a shared-memory tile.
 //
-// MI355X layout: a 64 x 4 pixel tile per 256-thread block (one wave per tile row, so
-// LDS reads of the window are consecutive words across lanes and conflict-free) with an
-// (r-1)-pixel clamp-to-edge halo staged once in LDS as FP32.  The FP16 LUT (rows of
+// MI355X layout: a TX x TY pixel tile per block of TX * TY threads (64 x 4 by default: one
+// wave per tile row, so LDS reads of the window are consecutive words across lanes) with
+// an (r-1)-pixel clamp-to-edge halo staged once in LDS as FP32.  The FP16 LUT (rows of
 // ceil(n^2/2) RGBA texels) stays in HBM and is served from L1/L2: 13 texels x 8 B per
 // pixel for r3.  Outputs are written as two horizontal pairs per thread (rows 2y, 2y+1),
 // coalesced across the wave.
@@ -15,15 +15,13 @@
 
 namespace mpx {
 
-constexpr int LITE_TX = 64;
-constexpr int LITE_TY = 4;
-
-template <int R, bool AR>
-__global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __restrict__ lut, int total) {
+template <int R, bool AR, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void ravu_lite_kernel(Io io, const __half* __restrict__ lut, int total) {
+  constexpr int NT = TX * TY;  // threads per block
   constexpr int N = 2 * R - 1;
   constexpr int HALO = R - 1;
-  constexpr int PW = LITE_TX + 2 * HALO;
-  constexpr int PH = LITE_TY + 2 * HALO;
+  constexpr int PW = TX + 2 * HALO;
+  constexpr int PH = TY + 2 * HALO;
   constexpr int NTEX = (N * N + 1) / 2;
   constexpr int C = (N * N - 1) / 2;
   __shared__ float tile[PH * PW];
@@ -31,21 +29,21 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   // sample instead of once per pixel whose anti-ringing window reads it (up to 13)
   __shared__ float4 pw[AR ? PH * PW : 1];
 
-  const FrameTile ft = frame_tile<LITE_TX, LITE_TY>(io, blockIdx.x, total);
+  const FrameTile ft = frame_tile<TX, TY>(io, blockIdx.x, total);
   const int frame = ft.f;
   const int x0 = ft.x0;
   const int y0 = ft.y0;
   const void* in = frame_in(io, frame);
 
   {
-    HaloTile<1, PH, PW, 256> h;
+    HaloTile<1, PH, PW, NT> h;
     h.issue(in, io.in_dt, ft.in_pitch, 0, ft.W, ft.H, x0 - HALO, y0 - HALO);
     h.commit(tile, io.in_dt, io.in_max, io.in_rmax);
   }
   if constexpr (AR) {
     // each thread takes the samples it staged itself (the commit's index pattern), so the
     // one block barrier below orders both tables
-    for (int i = threadIdx.x; i < PH * PW; i += 256) {
+    for (int i = threadIdx.x; i < PH * PW; i += NT) {
       const float l = tile[i];
       const float p0 = 0.1f + l, m0 = 1.1f - l;
       float p32 = p0, m32 = m0;
@@ -56,8 +54,8 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   }
   __syncthreads();
 
-  const int tx = threadIdx.x % LITE_TX;
-  const int ty = threadIdx.x / LITE_TX;
+  const int tx = threadIdx.x % TX;
+  const int ty = threadIdx.x / TX;
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
 
@@ -163,16 +161,28 @@ __global__ __launch_bounds__(256) void ravu_lite_kernel(Io io, const __half* __r
   store_pair(out, o0 + ft.out_pitch, io.out_dt, r4[1], r4[3]);
 }
 
-hipError_t launch_ravu_lite(int r, bool ar, const Io& io_, const __half* lut, hipStream_t st) {
-  Io io = io_;
-  const int total = launch_tiles<LITE_TX, LITE_TY>(io);
-  dim3 grid(total);
-  dim3 block(LITE_TX * LITE_TY);
-#define MPX_LITE(R, A) \
-  if (r == R && ar == A) { hipLaunchKernelGGL((ravu_lite_kernel<R, A>), grid, block, 0, st, io, lut, total); return hipGetLastError(); }
+template <int T>
+static hipError_t launch_lite_tile(int r, bool ar, Io io, const __half* lut, hipStream_t st) {
+  constexpr int TX = kLiteTiles[T].w, TY = kLiteTiles[T].h;
+  const int total = launch_tiles<TX, TY>(io);
+#define MPX_LITE(R, A)                                                                                  \
+  if (r == R && ar == A) {                                                                              \
+    hipLaunchKernelGGL((ravu_lite_kernel<R, A, TX, TY>), dim3(total), dim3(TX * TY), 0, st, io, lut, total); \
+    return hipGetLastError();                                                                           \
+  }
   MPX_LITE(2, false) MPX_LITE(2, true) MPX_LITE(3, false) MPX_LITE(3, true) MPX_LITE(4, false) MPX_LITE(4, true)
 #undef MPX_LITE
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_ravu_lite(int r, bool ar, const Io& io, const __half* lut, int tile, hipStream_t st) {
+  static_assert(kLiteTileCount == 3, "one case per tile shape");
+  switch (tile) {
+    case 0: return launch_lite_tile<0>(r, ar, io, lut, st);
+    case 1: return launch_lite_tile<1>(r, ar, io, lut, st);
+    case 2: return launch_lite_tile<2>(r, ar, io, lut, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace mpx

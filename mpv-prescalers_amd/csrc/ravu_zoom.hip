@@ -5,8 +5,9 @@
 // Per output pixel o: pos = (o + 1/2) * in/out (OFFSET ALIGN); subpix f = fract(pos - 1/2);
 // base = pos - f.  The window is the 2r x 2r input block at offsets -(r-1)..r around base.
 // The hash depends on the base pixel only, so a block first hashes every base pixel its
-// 32 x 8 output tile touches (into LDS), then each output pixel reads its base's bucket;
-// AR variants also tabulate the per-sample powers of the anti-ringing weights there.
+// output tile (32 x 8 by default) touches (into LDS), then each output pixel reads its
+// base's bucket; AR variants also tabulate the per-sample powers of the anti-ringing
+// weights there.
 // Base and phase are computed in exact integer arithmetic:
 //   pos - 1/2 = ((2o+1)*in - out) / (2*out)  ->  base = floor, f = remainder / (2*out),
 // once per output column and row at plan creation (zoom_phase_kernel); the kernel reads
@@ -25,10 +26,6 @@
 
 namespace mpx {
 
-constexpr int ZT_X = 32;
-constexpr int ZT_Y = 8;
-constexpr int ZB_X = ZT_X + 1;  // base pixels an output tile can touch (ratio in/out < 1)
-constexpr int ZB_Y = ZT_Y + 1;
 
 __device__ __forceinline__ int floor_div(int64_t a, int64_t b) {  // b > 0
   int64_t q = a / b;
@@ -111,28 +108,29 @@ __device__ __attribute__((noinline)) int zoom_redecide(const float* t, int o) {
 // the area's row stride PW.  AR forms store each sample's anti-ringing powers beside it
 // ((0.1 + l)^32, (1.1 - l)^32 and the ^33 terms, once per sample instead of once per
 // output pixel that reads it).  Sample i of the dense P x uh x uw order is split with
-// multiply-shift quotients, exact for every i < P * PN (i * d < 2^20 for both divisors).
-template <int P, int PH, int PW, bool AR>
+// high-half multiply quotients: floor(i / d) = mulhi(i, floor((2^32 - 1) / d) + 1), exact
+// while i * d < 2^32.
+template <int P, int PH, int PW, int NT, bool AR>
 __device__ __forceinline__ void stage_used(float* tile, float4* pw, const void* base, int dt, float in_max,
                                            float in_rmax, int64_t pitch, int64_t cstride, int W, int H, int ox,
                                            int oy, int uw, int uh) {
-  constexpr int PN = PH * PW, E = (P * PN + 255) / 256;
-  static_assert(P * PN * PN < (1 << 20) && PN * PW < (1 << 20), "multiply-shift quotients exact");
+  constexpr int PN = PH * PW, E = (P * PN + NT - 1) / NT;
+  static_assert(static_cast<int64_t>(P) * PN * PN < (int64_t{1} << 32), "high-half quotients exact");
   const int un = uw * uh, total = P * un;
-  const uint32_t mn = (1u << 20) / static_cast<uint32_t>(un) + 1u;
-  const uint32_t mw = (1u << 20) / static_cast<uint32_t>(uw) + 1u;
+  const uint32_t mn = 0xffffffffu / static_cast<uint32_t>(un) + 1u;
+  const uint32_t mw = 0xffffffffu / static_cast<uint32_t>(uw) + 1u;
   const int lsh = dt == DT_F32 ? 2 : (dt == DT_U8 ? 0 : 1);  // log2 element bytes
   const uintptr_t b = reinterpret_cast<uintptr_t>(base);
   uint32_t v[E], sh[E];
   int li[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int i = static_cast<int>(threadIdx.x) + 256 * e;
+    const int i = static_cast<int>(threadIdx.x) + NT * e;
     li[e] = -1;
     if (i < total) {
-      const int c = P == 1 ? 0 : static_cast<int>((static_cast<uint32_t>(i) * mn) >> 20);
+      const int c = P == 1 ? 0 : static_cast<int>(__umulhi(static_cast<uint32_t>(i), mn));
       const int j = i - c * un;
-      const int r = static_cast<int>((static_cast<uint32_t>(j) * mw) >> 20);
+      const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(j), mw));
       const int col = j - r * uw;
       const int gx = min(max(ox + col, 0), W - 1), gy = min(max(oy + r, 0), H - 1);
       const uintptr_t a = b + (static_cast<uintptr_t>(c * cstride + static_cast<int64_t>(gy) * pitch + gx) << lsh);
@@ -213,11 +211,12 @@ __global__ __launch_bounds__(256) void zoom_phase_kernel(int4* tab, int in_w, in
   tab[i] = make_int4(ph.base, __float_as_int(ph.f), o % (x ? nphx : nphy), 0);
 }
 
-// (kernels without anti-ringing: at least 6 waves per SIMD, a register budget their
-// out-of-line re-decision inherits.)  TAB: the plan has the pre-interpolated weight table;
-// separate instances keep the per-pixel bilinear fetch's registers out of the table path.
-template <int R, bool AR, int CH, bool TAB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6))) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
+// One block per TX x TY output tile, one thread per output pixel (kZoomTiles).  (Kernels
+// without anti-ringing: at least 6 waves per SIMD, a register budget their out-of-line
+// re-decision inherits.)  TAB: the plan has the pre-interpolated weight table; separate
+// instances keep the per-pixel bilinear fetch's registers out of the table path.
+template <int R, bool AR, int CH, bool TAB, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6))) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
                                                         const __half* __restrict__ lut_ar,
                                                         const float4* __restrict__ wtab, int nphx, int nphy,
                                                         const int4* __restrict__ phase, int out_w, int out_h) {
@@ -227,24 +226,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
   constexpr int GROUPS = (HALF + 3) / 4;
   constexpr int WIDTH = 9 * GROUPS;      // LUT texels per row
   constexpr int ROWS = 2592;             // 288 buckets x 9 phases
-  constexpr int PW = ZB_X + N - 1, PH = ZB_Y + N - 1, PN = PW * PH;
+  constexpr int BX = TX + 1, BY = TY + 1;  // base pixels a tile can touch (ratio in/out < 1)
+  constexpr int PW = BX + N - 1, PH = BY + N - 1, PN = PW * PH;
   __shared__ float tile[P * PN];
-  __shared__ int bucket[ZB_Y * ZB_X];
+  __shared__ int bucket[BY * BX];
   // AR: per staged sample (l + 0.1)^32, (1.1 - l)^32 and the ^33 terms, computed once per
   // sample instead of once per output pixel that reads it (about 25 at 2x)
   __shared__ float4 pw[AR ? P * PN : 1];
 
-  const BlockXYZ bk3 = block_xyz((out_w + ZT_X - 1) / ZT_X, (out_h + ZT_Y - 1) / ZT_Y, io.batch);
+  const BlockXYZ bk3 = block_xyz((out_w + TX - 1) / TX, (out_h + TY - 1) / TY, io.batch);
   const int f = bk3.z;
-  const int ox0 = bk3.x * ZT_X, oy0 = bk3.y * ZT_Y;
+  const int ox0 = bk3.x * TX, oy0 = bk3.y * TY;
   const int4* phy_tab = phase + out_w;
   const int bx0 = phase[ox0].x;
   const int by0 = phy_tab[oy0].x;
-  // base pixels this tile's (in-range) outputs touch: nbx x nby <= ZB_X x ZB_Y (in < out)
-  const int nbx = phase[min(ox0 + ZT_X, out_w) - 1].x - bx0 + 1;
-  const int nby = phy_tab[min(oy0 + ZT_Y, out_h) - 1].x - by0 + 1;
+  // base pixels this tile's (in-range) outputs touch: nbx x nby <= BX x BY (in < out)
+  const int nbx = phase[min(ox0 + TX, out_w) - 1].x - bx0 + 1;
+  const int nby = phy_tab[min(oy0 + TY, out_h) - 1].x - by0 + 1;
   const void* in = frame_in(io, f);
-  stage_used<P, PH, PW, AR>(tile, pw, in, io.in_dt, io.in_max, io.in_rmax, io.in_pitch, io.in_cstride, io.W, io.H,
+  stage_used<P, PH, PW, TX * TY, AR>(tile, pw, in, io.in_dt, io.in_max, io.in_rmax, io.in_pitch, io.in_cstride, io.W, io.H,
                             bx0 - (R - 1), by0 - (R - 1), nbx + N - 1, nby + N - 1);
   __syncthreads();
   // hash every base pixel the tile's outputs use (unclamped LUT row: bucket 9-row blocks
@@ -257,24 +257,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
       // inline, in the lean tensor forms for r3 (r3 rgb: 216 -> 123 VGPRs)
       const Bucket bk = ravu_hash<N, true, RULE_LITE, (R >= 3)>([&](int s) { return zluma_f<CH>(tile, off(s), PN); },
                                                       [&](int s) { return zluma_d<CH>(tile, off(s), PN); });
-      bucket[by * ZB_X + bx] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
+      bucket[by * BX + bx] = (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
     } else {
       float ta, tb, td;
       structure_tensor<N, true, float, (R >= 3)>([&](int s) { return zluma_f<CH>(tile, off(s), PN); }, ta, tb, td);
       bool near = false;
       const Bucket bk = hash_fast<RULE_LITE>(ta, tb, td, &near);
-      bucket[by * ZB_X + bx] = near ? zoom_redecide<N, CH, PW, PN, TAB>(tile, off(0))
+      bucket[by * BX + bx] = near ? zoom_redecide<N, CH, PW, PN, TAB>(tile, off(0))
                                     : (bk.angle * 4 + bk.strength) * 3 + bk.coherence;
     }
   }
   __syncthreads();
 
-  const int ox = ox0 + threadIdx.x % ZT_X, oy = oy0 + threadIdx.x / ZT_X;
+  const int ox = ox0 + threadIdx.x % TX, oy = oy0 + threadIdx.x / TX;
   if (ox >= out_w || oy >= out_h) return;
   const int4 phx = phase[ox], phy = phy_tab[oy];
   const float fx = __int_as_float(phx.y), fy = __int_as_float(phy.y);
   const int lbx = phx.x - bx0, lby = phy.x - by0;  // in [0, nbx) x [0, nby) by construction
-  const int row = bucket[lby * ZB_X + lbx];
+  const int row = bucket[lby * BX + lbx];
   if (io.buckets) io.buckets[f * io.bucket_bstride + static_cast<int64_t>(oy) * out_w + ox] = row;
   auto si = [&](int c, int i) { return c * PN + (lby + i % N) * PW + lbx + i / N; };
   const float txf = 8.f * fx, txi = 8.f * (1.f - fx);
@@ -359,26 +359,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR ? 1 : 6)
   }
 }
 
-hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
-                            const ZoomPlan& zp, int out_w, int out_h, hipStream_t st) {
-  if (!zp.phase) return hipErrorInvalidValue;
-  dim3 grid(((out_w + ZT_X - 1) / ZT_X) * ((out_h + ZT_Y - 1) / ZT_Y) * io.batch);
-#define MPX_ZOOM(R, A, CH)                                                                             \
-  if (r == R && ar == A && channels == CH) {                                                           \
-    if (zp.wtab)                                                                                       \
-      hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, true>), grid, dim3(256), 0, st, io, lut, lut_ar,    \
-                         reinterpret_cast<const float4*>(zp.wtab), zp.nphx, zp.nphy, zp.phase, out_w,    \
-                         out_h);                                                                         \
-    else                                                                                               \
-      hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, false>), grid, dim3(256), 0, st, io, lut, lut_ar,   \
-                         nullptr, 0, 0, zp.phase, out_w, out_h);                                         \
-    return hipGetLastError();                                                                          \
+template <int T>
+static hipError_t launch_zoom_tile(int r, bool ar, int channels, const Io& io, const __half* lut,
+                                   const __half* lut_ar, const ZoomPlan& zp, int out_w, int out_h, hipStream_t st) {
+  constexpr int TX = kZoomTiles[T].w, TY = kZoomTiles[T].h;
+  const dim3 grid(((out_w + TX - 1) / TX) * ((out_h + TY - 1) / TY) * io.batch), block(TX * TY);
+  const float4* wtab = reinterpret_cast<const float4*>(zp.wtab);
+#define MPX_ZOOM(R, A, CH)                                                                                   \
+  if (r == R && ar == A && channels == CH) {                                                                 \
+    if (zp.wtab)                                                                                             \
+      hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, true, TX, TY>), grid, block, 0, st, io, lut, lut_ar, wtab, \
+                         zp.nphx, zp.nphy, zp.phase, out_w, out_h);                                          \
+    else                                                                                                     \
+      hipLaunchKernelGGL((ravu_zoom_kernel<R, A, CH, false, TX, TY>), grid, block, 0, st, io, lut, lut_ar,    \
+                         nullptr, 0, 0, zp.phase, out_w, out_h);                                             \
+    return hipGetLastError();                                                                                \
   }
   MPX_ZOOM(2, false, 0) MPX_ZOOM(2, false, 1) MPX_ZOOM(2, false, 2) MPX_ZOOM(3, false, 0) MPX_ZOOM(3, false, 1)
   MPX_ZOOM(3, false, 2) MPX_ZOOM(2, true, 0) MPX_ZOOM(2, true, 1) MPX_ZOOM(2, true, 2) MPX_ZOOM(3, true, 0)
   MPX_ZOOM(3, true, 1) MPX_ZOOM(3, true, 2)
 #undef MPX_ZOOM
   return hipErrorInvalidValue;
+}
+
+hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,
+                            const ZoomPlan& zp, int out_w, int out_h, int tile, hipStream_t st) {
+  if (!zp.phase) return hipErrorInvalidValue;
+  static_assert(kZoomTileCount == 2, "one case per tile shape");
+  switch (tile) {
+    case 0: return launch_zoom_tile<0>(r, ar, channels, io, lut, lut_ar, zp, out_w, out_h, st);
+    case 1: return launch_zoom_tile<1>(r, ar, channels, io, lut, lut_ar, zp, out_w, out_h, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 int zoom_plan_create(mpx_ctx* ctx, mpx_plan* plan, int in_w, int in_h, int out_w, int out_h) {

@@ -25,7 +25,8 @@ CHANNELS = {0: "luma", 1: "yuv", 2: "rgb"}
 # every exported symbol declared in include/mpx.h
 EXPORTS = ("mpx_variant_count", "mpx_variant_name", "mpx_query", "mpx_create", "mpx_destroy", "mpx_ctx_bytes",
            "mpx_plan_create", "mpx_plan_info", "mpx_plan_bucket_count", "mpx_plan_destroy", "mpx_run",
-           "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_plan_set_shape", "mpx_resample",
+           "mpx_run_ex", "mpx_run_frames", "mpx_run_frames_sized", "mpx_plan_tune", "mpx_plan_set_shape",
+           "mpx_plan_set_tile", "mpx_plan_tile", "mpx_resample",
            "mpx_weights_verify", "mpx_run_static", "mpx_queue_create", "mpx_queue_destroy", "mpx_submit", "mpx_poll",
            "mpx_queue_launches", "mpx_queue_hold", "mpx_last_error")
 MAX_FRAMES = 16
@@ -97,6 +98,9 @@ def _load():
         "mpx_plan_tune": (i32, [vp, ctypes.POINTER(MpxIo), vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                 ctypes.POINTER(ctypes.c_float)]),
         "mpx_plan_set_shape": (i32, [vp, i32, i32]),
+        "mpx_plan_set_tile": (i32, [vp, i32]),
+        "mpx_plan_tile": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                ctypes.POINTER(i32)]),
         "mpx_resample": (i32, [ctypes.POINTER(MpxIo), i32, i32, i32, i32, i32, i32, i32, i32, i32,
                                ctypes.c_float, ctypes.c_float, vp]),
         "mpx_weights_verify": (i32, [ctypes.c_char_p, ctypes.c_char_p]),

@@ -100,20 +100,34 @@ class Plan:
 
     def tune(self, x, out=None, iters=5, stream=None):
         """Autotune the launch shape on ``x`` / ``out`` (writes ``out``); the plan keeps the
-        fastest.  Returns {"grid_per_cu", "split", "us"} (grid 0 = occupancy maximum;
-        plans without a tunable shape return zeros)."""
+        fastest.  Returns {"grid_per_cu", "split", "tile", "us"}: NNEDI3 MFMA plans tune the
+        persistent grid (0 = occupancy maximum) and neuron split, RAVU plans the tile
+        shape of their kernels (index into the family's list); plans without a tunable
+        shape return zeros."""
         io, x, out, _ = self._io(x, out, False)
         if stream is None:
             stream = torch.cuda.current_stream(x.device)
         g, sp, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         check(LIB.mpx_plan_tune(self._h, ctypes.byref(io), ctypes.c_void_p(stream.cuda_stream), iters,
                                 ctypes.byref(g), ctypes.byref(sp), ctypes.byref(us)))
-        return {"grid_per_cu": g.value, "split": sp.value, "us": us.value}
+        return {"grid_per_cu": g.value, "split": sp.value, "tile": self.tile()["tile"], "us": us.value}
 
     def set_shape(self, grid_per_cu=0, split=0):
         """Set the launch shape directly (e.g. a cached ``tune`` result): blocks per CU
         (0 = occupancy maximum) and, for NNEDI3 nns 64 / 128, the neuron split."""
         check(LIB.mpx_plan_set_shape(self._h, int(grid_per_cu), int(split)))
+
+    def set_tile(self, tile):
+        """RAVU plans: launch their kernels on tile shape ``tile`` of the family's list
+        (0 = default; results are the same for every shape)."""
+        check(LIB.mpx_plan_set_tile(self._h, int(tile)))
+
+    def tile(self):
+        """{"tile", "count", "w", "h"}: the current tile shape of a RAVU plan's kernels
+        (pixels per block) and the number of candidates (1, size 0 x 0, for NNEDI3)."""
+        t, n, w, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(LIB.mpx_plan_tile(self._h, ctypes.byref(t), ctypes.byref(n), ctypes.byref(w), ctypes.byref(h)))
+        return {"tile": t.value, "count": n.value, "w": w.value, "h": h.value}
 
     def _check_frame(self, x):
         """One frame of run_frames / run_frames_sized / the serving loop: a contiguous

@@ -26,11 +26,30 @@ def test_tune_nnedi3(nns, wh):
 
 
 def test_tune_other_plans_untouched():
+    """NNEDI3 FP32 plans have no tunable shape: tune runs nothing and reports zeros."""
     import mpx
     x = torch.rand((32, 48), device="cuda")
-    plan = mpx.Context(0).plan("ravu-3x-r2", 48, 32)
+    plan = mpx.Context(0).plan("nnedi3-nns16-win8x4", 48, 32)
     before = plan.run(x).clone()
-    assert plan.tune(x) == {"grid_per_cu": 0, "split": 0, "us": 0.0}
+    assert plan.tune(x) == {"grid_per_cu": 0, "split": 0, "tile": 0, "us": 0.0}
+    assert plan.tile() == {"tile": 0, "count": 1, "w": 0, "h": 0}
+    assert torch.equal(plan.run(x), before)
+
+
+@pytest.mark.parametrize("name", ["ravu-lite-ar-r3", "ravu-3x-r3-yuv", "ravu-r3-rgb", "ravu-zoom-ar-r2"])
+def test_tune_ravu_tile(name):
+    """RAVU plans tune their tile shape: the choice is kept and the result is unchanged."""
+    import mpx
+    from mpx.frames import make_frame
+    planes = 3 if name.endswith(("-yuv", "-rgb")) else 1
+    x = torch.from_numpy(make_frame(320, 180, 3, bits=10, channels=planes).astype(np.float16)).cuda()
+    plan = mpx.Context(0).plan(name, 320, 180, in_dtype=torch.float16, out_dtype=torch.float16,
+                               target=(480, 270) if "zoom" in name else None)
+    before = plan.run(x).clone()
+    res = plan.tune(x, iters=3)
+    t = plan.tile()
+    assert res["grid_per_cu"] == 0 and res["split"] == 0 and res["us"] > 0
+    assert res["tile"] == t["tile"] and 0 <= t["tile"] < t["count"] and t["count"] >= 2 and t["w"] * t["h"] >= 128
     assert torch.equal(plan.run(x), before)
 
 
@@ -63,4 +82,11 @@ def test_set_shape_rejects():
     lite = ctx.plan("ravu-lite-r3", 64, 32)
     lite.set_shape(0, 0)
     with pytest.raises(MpxError):
-        lite.set_shape(2, 0)          # no tunable shape
+        lite.set_shape(2, 0)          # no grid / split to set
+    with pytest.raises(MpxError):
+        lite.set_tile(lite.tile()["count"])
+    with pytest.raises(MpxError):
+        lite.set_tile(-1)
+    with pytest.raises(MpxError):
+        nn32.set_tile(1)              # NNEDI3 plans have one tile shape
+    nn32.set_tile(0)

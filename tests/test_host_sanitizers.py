@@ -66,6 +66,7 @@ int main(int argc, char** argv) {
   if (mpx_plan_info(NULL, &info) != MPX_ERR_ARG || mpx_plan_bucket_count(NULL) != 0) return 16;
   if (mpx_plan_tune(NULL, &io, NULL, 1, NULL, NULL, NULL) != MPX_ERR_ARG) return 17;
   if (mpx_plan_set_shape(NULL, 0, 0) != MPX_ERR_ARG || mpx_ctx_bytes(NULL) != 0) return 18;
+  if (mpx_plan_set_tile(NULL, 0) != MPX_ERR_ARG || mpx_plan_tile(NULL, NULL, NULL, NULL, NULL) != MPX_ERR_ARG) return 25;
   mpx_plan_destroy(NULL);
   mpx_destroy(NULL);
   mpx_queue* q = NULL;
