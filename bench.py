@@ -71,15 +71,24 @@ def zoom_ar3_weights_dir(tmp):
     return tmp
 
 
-def lds_bytes_per_frame(key):
-    """LDS bytes one frame of a RAVU secondary workload moves (hardware counters,
-    profiles/counters_ravu_lds.json), or None."""
+def lds_bytes_per_frame(key, tile):
+    """LDS bytes one frame of a RAVU secondary workload moves at tile shape `tile`
+    (hardware counters, profiles/counters_ravu_lds.json), or None."""
     path = os.path.join(ROOT, "profiles", "counters_ravu_lds.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         w = json.load(f)["workloads"].get(key)
-    return w["lds_bytes_per_frame"] if w else None
+    return w["lds_bytes_per_frame"] if w and w.get("tile") == list(tile) else None
+
+
+def tuned_tile(pl, xin, b, torch):
+    """Tile shape the autotuner picks for plan `pl` on `xin` (b frames) and its time per
+    frame, timed like the line's main figure."""
+    pl.tune(xin, iters=3)
+    t = pl.tile()
+    _, wall = timed(lambda: pl.run(xin), 10, 2, torch)
+    return {"tile": [t["w"], t["h"]], "us_per_frame": round(wall / (10 * b) * 1e6, 2)}
 
 
 def nnedi3_flop_per_px(name):
@@ -430,20 +439,29 @@ def kernel_only_nns256(ctx, steps, torch):
 
 
 def kernel_only_ravu(ctx, steps, torch):
-    """The RAVU variants of the secondary list, `steps` launches each."""
+    """The RAVU variants of the secondary list, `steps` launches each, at their family's
+    default tile shape (the shape the secondary lines' main figures use; a tuner run under
+    a profiler would compare distorted timings).  Before each variant's launches a
+    one-element fill kernel is launched: in a kernel trace or counter run the dispatches
+    between two fills are exactly that variant's `steps` launches."""
     import numpy as np
     from mpx.frames import make_plane
-    for name, (iw, ih), tgt, planes, b, dt in RAVU_SECONDARY:
+    tiles = []
+    for i, (name, (iw, ih), tgt, planes, b, dt) in enumerate(RAVU_SECONDARY):
         dt = getattr(torch, dt)
         fr = np.stack([make_plane(iw, ih, seed=i) for i in range(b * planes)]).astype(np.float32)
         xin = torch.from_numpy(fr.reshape((b, planes, ih, iw) if planes == 3 else (b, ih, iw))).to(dt).to("cuda:0")
         pl = ctx.plan(name, iw, ih, target=tgt, in_dtype=dt, out_dtype=dt, max_batch=b)
         out = pl.run(xin)
-        pl.tune(xin, out, iters=3)   # the tile shape the bench lines use
+        t = pl.tile()
+        tiles.append([t["w"], t["h"]])
+        torch.full((1,), float(i), device="cuda:0")   # boundary marker
         for _ in range(steps):
             pl.run(xin, out=out)
+    torch.full((1,), -1.0, device="cuda:0")
     torch.cuda.synchronize()
-    print(json.dumps({"workload": [r[0] for r in RAVU_SECONDARY], "launches_each": steps}), flush=True)
+    print(json.dumps({"workload": [r[0] for r in RAVU_SECONDARY], "launches_each": steps, "tiles": tiles}),
+          flush=True)
     return 0
 
 
@@ -532,8 +550,7 @@ def main():
             fr = np.stack([make_plane(iw, ih, seed=i) for i in range(b * planes)]).astype(np.float32)
             xin = torch.from_numpy(fr.reshape((b, planes, ih, iw) if planes == 3 else (b, ih, iw))).to(dt).to(dev)
             pl = ctx.plan(name, iw, ih, target=tgt, in_dtype=dt, out_dtype=dt, max_batch=b)
-            pl.tune(xin, iters=3)                        # tile shape, kept by the plan
-            tile = pl.tile()
+            tile = pl.tile()                             # the family's default tile shape
             t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
             opx = pl.info["out_w"] * pl.info["out_h"]
             io_bytes = (iw * ih + opx) * planes * (4 if dt == torch.float32 else 2)
@@ -544,11 +561,12 @@ def main():
                 "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
                 "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4),
                 "tile": [tile["w"], tile["h"]]}
-            lds = lds_bytes_per_frame(key)
+            lds = lds_bytes_per_frame(key, (tile["w"], tile["h"]))
             if lds:
                 # LDS traffic per frame from the hardware counters of the same workload
                 secondary[key]["lds_gb_s"] = round(b * 10 * lds / wall / 1e9, 1)
                 secondary[key]["lds_frac"] = round(b * 10 * lds / wall / 1e9 / PEAK_LDS, 4)
+            secondary[key]["tuned"] = tuned_tile(pl, xin, b, torch)
 
         # ravu-zoom-ar-r3-rgb at 2x and 1.5x on a supplied (synthetic) anti-ringing table
         import tempfile
@@ -559,7 +577,6 @@ def main():
                 xin = torch.from_numpy(fr.reshape(b, 3, ih, iw)).to(dev)
                 pl = zctx.plan("ravu-zoom-ar-r3-rgb", iw, ih, target=(tw, th), in_dtype=torch.float16,
                                out_dtype=torch.float16, max_batch=b)
-                pl.tune(xin, iters=3)
                 tile = pl.tile()
                 t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
                 io_bytes = (iw * ih + tw * th) * 3 * 2
@@ -568,6 +585,7 @@ def main():
                        "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4),
                        "tile": [tile["w"], tile["h"]],
                        "weights": "synthetic AR table (|r3 main table|; the reference ships none)"}
+                rec["tuned"] = tuned_tile(pl, xin, b, torch)
                 err = parity_record("ravu-zoom-ar-r3-rgb %dx%d->%dx%d" % (iw, ih, tw, th))
                 if err:
                     rec["error_vs_fp64_oracle"] = err
