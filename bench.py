@@ -442,8 +442,9 @@ def kernel_only_ravu(ctx, steps, torch):
     """The RAVU variants of the secondary list, `steps` launches each, at their family's
     default tile shape (the shape the secondary lines' main figures use; a tuner run under
     a profiler would compare distorted timings).  Before each variant's launches a
-    one-element fill kernel is launched: in a kernel trace or counter run the dispatches
-    between two fills are exactly that variant's `steps` launches."""
+    one-element fill kernel is launched and another after them: in a kernel trace or
+    counter run the dispatches between those two fills are exactly that variant's `steps`
+    launches."""
     import numpy as np
     from mpx.frames import make_plane
     tiles = []
@@ -455,10 +456,10 @@ def kernel_only_ravu(ctx, steps, torch):
         out = pl.run(xin)
         t = pl.tile()
         tiles.append([t["w"], t["h"]])
-        torch.full((1,), float(i), device="cuda:0")   # boundary marker
+        torch.full((1,), float(i), device="cuda:0")   # start marker
         for _ in range(steps):
             pl.run(xin, out=out)
-    torch.full((1,), -1.0, device="cuda:0")
+        torch.full((1,), -1.0, device="cuda:0")       # end marker
     torch.cuda.synchronize()
     print(json.dumps({"workload": [r[0] for r in RAVU_SECONDARY], "launches_each": steps, "tiles": tiles}),
           flush=True)
