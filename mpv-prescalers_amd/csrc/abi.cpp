@@ -139,6 +139,18 @@ int elem_size(int t) { return t == MPX_F32 ? 4 : (t == MPX_U8 ? 1 : 2); }
 // Kernels index tiles and blocks of one launch with 32-bit integers.
 constexpr int64_t kMaxLaunchPixels = (int64_t{1} << 31) - 1;
 
+// The staging loaders address an input frame's samples with 32-bit element offsets (plane
+// * cstride + row * pitch + column, the row term a 24-bit multiply): a frame may span at
+// most 2^32 elements (16 GiB of FP32) with a pitch below 2^24 elements.
+constexpr int64_t kMaxSpan = int64_t{1} << 32;
+constexpr int64_t kMaxPitch = int64_t{1} << 24;
+int check_span(int64_t pitch, int64_t cstride, int planes, int h) {
+  const int64_t span = (planes > 1 ? (planes - 1) * cstride : 0) + static_cast<int64_t>(h) * pitch;
+  if (pitch >= kMaxPitch || cstride < 0 || span > kMaxSpan)
+    return fail(MPX_ERR_SHAPE, "input frame spans 2^32 elements or more, or its pitch 2^24 or more");
+  return MPX_OK;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------------------- static scratch
@@ -403,6 +415,7 @@ int mpx_host::run_io(mpx_plan* plan, const mpx_io* io, void* hip_stream, void* i
   if (io->buckets && !inf.applies) return fail(MPX_ERR_ARG, "buckets requested from a skipped plan");
   if (static_cast<int64_t>(inf.out_w) * inf.out_h * io->batch > kMaxLaunchPixels)
     return fail(MPX_ERR_SHAPE, "more than 2^31 output pixels per plane in one launch");
+  if (int rc = check_span(io->in_pitch, io->in_cstride, inf.planes, plan->in_h)) return rc;
   mpx_host::DeviceScope dscope;
   if (int drc = dscope.enter(plan->ctx->device)) return drc;
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
@@ -443,6 +456,7 @@ int mpx_run_frames(mpx_plan* plan, int n, const void* const* ins, void* const* o
     if (!ins[i] || !outs[i]) return fail(MPX_ERR_ARG, "frame " + std::to_string(i) + " has a NULL plane");
   if (static_cast<int64_t>(inf.out_w) * inf.out_h * n > kMaxLaunchPixels)
     return fail(MPX_ERR_SHAPE, "more than 2^31 output pixels per plane in one launch");
+  if (int rc = check_span(in_pitch, in_pitch * plan->in_h, inf.planes, plan->in_h)) return rc;
   mpx_host::DeviceScope dscope;
   if (int drc = dscope.enter(plan->ctx->device)) return drc;
   mpx::Io d{};
@@ -488,6 +502,7 @@ int mpx_run_frames_sized(mpx_plan* plan, int n, const void* const* ins, void* co
       return fail(MPX_ERR_SHAPE, "frame " + std::to_string(i) + " is not within the plan's input size");
     if (in_pitches[i] < widths[i] || out_pitches[i] < static_cast<int64_t>(widths[i]) * inf.scale_x)
       return fail(MPX_ERR_ARG, "frame " + std::to_string(i) + ": pitch smaller than the row width");
+    if (int rc = check_span(in_pitches[i], in_pitches[i] * heights[i], inf.planes, heights[i])) return rc;
     d.tab_in[i] = ins[i];
     d.tab_out[i] = outs[i];
     d.fw[i] = widths[i];

@@ -156,6 +156,9 @@ __device__ __forceinline__ float load_elem(const void* p, int64_t idx, int dt, f
 // leaves the sample's own page), its bit offset kept beside it.  `commit` extracts and
 // converts the samples and writes LDS.  A tile therefore costs one memory latency, and a
 // persistent block can issue its next tile while it computes this one.
+// Sample addresses are 32-bit element offsets from the frame (plane c * cstride + row *
+// pitch + column, the row term a 24-bit multiply): the host refuses frames that span 2^32
+// elements or more, or whose pitch reaches 2^24 elements (kMaxSpan, kMaxPitch).
 // A pointer into global memory (address space 1): loads through it are global loads, never
 // flat ones (which would also count against the LDS wait counter).
 typedef const __attribute__((address_space(1))) uint32_t* GlobalU32;
@@ -172,17 +175,19 @@ struct HaloTile {
                                         int ox, int oy) {
     const int lsh = dt == DT_F32 ? 2 : (dt == DT_U8 ? 0 : 1);  // log2 element bytes
     const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+    const uint32_t ps = static_cast<uint32_t>(pitch), cs = static_cast<uint32_t>(cstride);
     if (ox >= 0 && oy >= 0 && ox + PW <= W && oy + PH <= H) {
       // interior tile (uniform branch): no clamping, and the per-thread part of each
-      // address (c*cstride + r*pitch + col) does not depend on the tile
-      const uintptr_t t0 = b + (static_cast<uintptr_t>(static_cast<int64_t>(oy) * pitch + ox) << lsh);
+      // offset (c*cstride + r*pitch + col) does not depend on the tile
+      const uintptr_t t0 = b + (static_cast<uintptr_t>(__umul24(static_cast<uint32_t>(oy), ps) + ox) << lsh);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = min(static_cast<int>(threadIdx.x) + e * NT, N - 1);
         const int c = P == 1 ? 0 : i / PN;
         const int j = i - c * PN;
         const int r = j / PW, col = j - r * PW;
-        const uintptr_t a = t0 + (static_cast<uintptr_t>(c * cstride + r * pitch + col) << lsh);
+        const uint32_t off = static_cast<uint32_t>(c) * cs + __umul24(static_cast<uint32_t>(r), ps) + col;
+        const uintptr_t a = t0 + (static_cast<uintptr_t>(off) << lsh);
         v[e] = *reinterpret_cast<GlobalU32>(a & ~static_cast<uintptr_t>(3));
         sh[e] = static_cast<uint32_t>(a & 3) * 8;
       }
@@ -196,7 +201,8 @@ struct HaloTile {
       const int r = j / PW, col = j - r * PW;
       const int gx = min(max(ox + col, 0), W - 1);
       const int gy = min(max(oy + r, 0), H - 1);
-      const uintptr_t a = b + (static_cast<uintptr_t>(c * cstride + static_cast<int64_t>(gy) * pitch + gx) << lsh);
+      const uint32_t off = static_cast<uint32_t>(c) * cs + __umul24(static_cast<uint32_t>(gy), ps) + gx;
+      const uintptr_t a = b + (static_cast<uintptr_t>(off) << lsh);
       v[e] = *reinterpret_cast<GlobalU32>(a & ~static_cast<uintptr_t>(3));
       sh[e] = static_cast<uint32_t>(a & 3) * 8;
     }

@@ -133,7 +133,10 @@ __device__ __forceinline__ void stage_used(float* tile, float4* pw, const void* 
       const int r = static_cast<int>(__umulhi(static_cast<uint32_t>(j), mw));
       const int col = j - r * uw;
       const int gx = min(max(ox + col, 0), W - 1), gy = min(max(oy + r, 0), H - 1);
-      const uintptr_t a = b + (static_cast<uintptr_t>(c * cstride + static_cast<int64_t>(gy) * pitch + gx) << lsh);
+      // 32-bit element offset, as the halo loaders (the host bounds the frame's span)
+      const uint32_t off = static_cast<uint32_t>(c) * static_cast<uint32_t>(cstride) +
+                           __umul24(static_cast<uint32_t>(gy), static_cast<uint32_t>(pitch)) + gx;
+      const uintptr_t a = b + (static_cast<uintptr_t>(off) << lsh);
       v[e] = *reinterpret_cast<GlobalU32>(a & ~static_cast<uintptr_t>(3));
       sh[e] = static_cast<uint32_t>(a & 3) * 8;
       li[e] = c * PN + r * PW + col;

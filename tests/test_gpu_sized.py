@@ -101,3 +101,23 @@ def test_plan_refuses_another_stream_while_its_intermediates_are_busy():
     assert torch.equal(ob[0], oa[0])
     plan.close()
     ctx.close()
+
+
+def test_frame_span_limits_refused_before_launch():
+    """The staging loaders address samples with 32-bit element offsets: mpx_run refuses a
+    pitch of 2^24 elements or an input frame spanning more than 2^32 elements with
+    MPX_ERR_SHAPE before anything is launched (the buffers here are far smaller than the
+    strides claim, so a launch would read out of bounds)."""
+    import ctypes
+    import mpx
+    from mpx._abi import LIB
+    ctx = mpx.Context(0)
+    x = torch.zeros((600, 64), device="cuda")
+    o = torch.zeros((1200, 128), device="cuda")
+    plan = ctx.plan("ravu-lite-r2", 64, 600)
+    for pitch in (1 << 24, 1 << 23):     # pitch limit; 600 rows x 2^23 > 2^32 elements
+        rc = LIB.mpx_run(plan._h, ctypes.c_void_p(x.data_ptr()), pitch, pitch * 600, ctypes.c_void_p(o.data_ptr()),
+                         128, 128 * 1200, 1, None)
+        assert rc == -4, (pitch, rc)
+    torch.cuda.synchronize()
+    assert torch.equal(plan.run(x), mpx.upscale("ravu-lite-r2", x))
