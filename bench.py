@@ -28,6 +28,8 @@ PEAK_HBM = 8000.0   # GB/s
 PEAK_LDS = 256 * 128 * 2.4  # GB/s: 128 bytes per clock per CU at 2.4 GHz
 
 
+RAVU_STEPS = 30   # timed launches per RAVU secondary line (after 5 untimed)
+
 # (variant, input size, target, planes, batch, dtype name) of the RAVU secondary lines
 RAVU_SECONDARY = (("ravu-lite-ar-r3", (640, 360), None, 1, 64, "float32"),
                   ("ravu-lite-ar-r3", (1920, 1080), None, 1, 8, "float16"),
@@ -87,8 +89,8 @@ def tuned_tile(pl, xin, b, torch):
     frame, timed like the line's main figure."""
     pl.tune(xin, iters=3)
     t = pl.tile()
-    _, wall = timed(lambda: pl.run(xin), 10, 2, torch)
-    return {"tile": [t["w"], t["h"]], "us_per_frame": round(wall / (10 * b) * 1e6, 2)}
+    wall = wall_steps(lambda: pl.run(xin), RAVU_STEPS, 5, torch)
+    return {"tile": [t["w"], t["h"]], "us_per_frame": round(wall / (RAVU_STEPS * b) * 1e6, 2)}
 
 
 def nnedi3_flop_per_px(name):
@@ -113,6 +115,19 @@ def timed(fn, steps, warmup, torch):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return [s.elapsed_time(e) for s, e in zip(starts, ends)], wall
+
+
+def wall_steps(fn, steps, warmup, torch):
+    """Wall time of exactly `steps` back-to-back calls of fn() after `warmup` untimed ones,
+    synchronised on both sides, with nothing else enqueued in the timed region."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
 
 
 def region_ms(fn, steps, warmup, torch):
@@ -513,7 +528,11 @@ def main():
     res = {}
     for p, plan in plans.items():
         n = steps if p == "fp16" else max(10, steps // 10)
-        times, wall = timed(lambda: plan.run(x, out=out), n, warm if p == "fp16" else 3, torch)
+        # throughput: exactly n steps between synchronisations; latency percentiles from a
+        # separate pass with an event pair around each step (the event records would
+        # otherwise sit between the steps of the timed region)
+        wall = wall_steps(lambda: plan.run(x, out=out), n, warm if p == "fp16" else 3, torch)
+        times, _ = timed(lambda: plan.run(x, out=out), n, 0, torch)
         res[p] = {"ms_per_step": wall / n * 1e3, "times": times, "mpx_s": 4 * W * H * n / wall / 1e6}
 
     roofline = dominant_kernel(ctx, max(20, steps // 2), torch)
@@ -531,7 +550,7 @@ def main():
                           precision=p, max_batch=b)
             n = 2 if p == "fp32" else 5
             pl.tune(f10, o10, iters=2)
-            t, wall = timed(lambda: pl.run(f10, out=o10), n, 1, torch)
+            wall = wall_steps(lambda: pl.run(f10, out=o10), n, 1, torch)
             fps = b * n / wall
             rec = {"fps": round(fps, 1), "mpx_s": round(fps * 4 * W * H / 1e6, 1),
                    "tflops": round(fps * nnedi3_flop_per_px("nnedi3-nns256-win8x6") * 3 * W * H / 1e12, 1)}
@@ -552,21 +571,22 @@ def main():
             xin = torch.from_numpy(fr.reshape((b, planes, ih, iw) if planes == 3 else (b, ih, iw))).to(dt).to(dev)
             pl = ctx.plan(name, iw, ih, target=tgt, in_dtype=dt, out_dtype=dt, max_batch=b)
             tile = pl.tile()                             # the family's default tile shape
-            t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
+            wall = wall_steps(lambda: pl.run(xin), RAVU_STEPS, 5, torch)
             opx = pl.info["out_w"] * pl.info["out_h"]
             io_bytes = (iw * ih + opx) * planes * (4 if dt == torch.float32 else 2)
             key = "%s %dx%d b%d" % (name, iw, ih, b) if tgt is None or name.endswith("ar-r2-rgb") else \
                 "%s %dx%d->%dx%d b%d" % (name, iw, ih, tgt[0], tgt[1], b)
             secondary[key] = {
-                "mpx_s": round(b * 10 * opx / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
-                "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
-                "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4),
+                "mpx_s": round(b * RAVU_STEPS * opx / wall / 1e6, 1),
+                "us_per_frame": round(wall / (RAVU_STEPS * b) * 1e6, 2),
+                "hbm_gb_s": round(b * RAVU_STEPS * io_bytes / wall / 1e9, 1),
+                "hbm_frac": round(b * RAVU_STEPS * io_bytes / wall / 1e9 / PEAK_HBM, 4),
                 "tile": [tile["w"], tile["h"]]}
             lds = lds_bytes_per_frame(key, (tile["w"], tile["h"]))
             if lds:
                 # LDS traffic per frame from the hardware counters of the same workload
-                secondary[key]["lds_gb_s"] = round(b * 10 * lds / wall / 1e9, 1)
-                secondary[key]["lds_frac"] = round(b * 10 * lds / wall / 1e9 / PEAK_LDS, 4)
+                secondary[key]["lds_gb_s"] = round(b * RAVU_STEPS * lds / wall / 1e9, 1)
+                secondary[key]["lds_frac"] = round(b * RAVU_STEPS * lds / wall / 1e9 / PEAK_LDS, 4)
             secondary[key]["tuned"] = tuned_tile(pl, xin, b, torch)
 
         # ravu-zoom-ar-r3-rgb at 2x and 1.5x on a supplied (synthetic) anti-ringing table
@@ -579,11 +599,12 @@ def main():
                 pl = zctx.plan("ravu-zoom-ar-r3-rgb", iw, ih, target=(tw, th), in_dtype=torch.float16,
                                out_dtype=torch.float16, max_batch=b)
                 tile = pl.tile()
-                t, wall = timed(lambda: pl.run(xin), 10, 2, torch)
+                wall = wall_steps(lambda: pl.run(xin), RAVU_STEPS, 5, torch)
                 io_bytes = (iw * ih + tw * th) * 3 * 2
-                rec = {"mpx_s": round(b * 10 * tw * th / wall / 1e6, 1), "us_per_frame": round(wall / (10 * b) * 1e6, 2),
-                       "hbm_gb_s": round(b * 10 * io_bytes / wall / 1e9, 1),
-                       "hbm_frac": round(b * 10 * io_bytes / wall / 1e9 / PEAK_HBM, 4),
+                rec = {"mpx_s": round(b * RAVU_STEPS * tw * th / wall / 1e6, 1),
+                       "us_per_frame": round(wall / (RAVU_STEPS * b) * 1e6, 2),
+                       "hbm_gb_s": round(b * RAVU_STEPS * io_bytes / wall / 1e9, 1),
+                       "hbm_frac": round(b * RAVU_STEPS * io_bytes / wall / 1e9 / PEAK_HBM, 4),
                        "tile": [tile["w"], tile["h"]],
                        "weights": "synthetic AR table (|r3 main table|; the reference ships none)"}
                 rec["tuned"] = tuned_tile(pl, xin, b, torch)
