@@ -28,7 +28,7 @@ PEAK_HBM = 8000.0   # GB/s
 PEAK_LDS = 256 * 128 * 2.4  # GB/s: 128 bytes per clock per CU at 2.4 GHz
 
 
-RAVU_STEPS = 30   # timed launches per RAVU secondary line (after 5 untimed)
+RAVU_STEPS = 30   # timed launches per RAVU secondary line (after WARM_S seconds and 5 untimed)
 
 # (variant, input size, target, planes, batch, dtype name) of the RAVU secondary lines
 RAVU_SECONDARY = (("ravu-lite-ar-r3", (640, 360), None, 1, 64, "float32"),
@@ -117,9 +117,25 @@ def timed(fn, steps, warmup, torch):
     return [s.elapsed_time(e) for s, e in zip(starts, ends)], wall
 
 
+WARM_S = 0.3   # seconds of the same work before each timed region
+
+
+def warm(fn, torch, seconds=WARM_S):
+    """Run fn() for `seconds` (synchronising every 8 calls): the GPU idles while the bench
+    prepares each workload on the CPU, and a timed region that starts from idle clocks
+    measured up to 12 % slower than the same launches at steady state."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(8):
+            fn()
+        torch.cuda.synchronize()
+
+
 def wall_steps(fn, steps, warmup, torch):
-    """Wall time of exactly `steps` back-to-back calls of fn() after `warmup` untimed ones,
-    synchronised on both sides, with nothing else enqueued in the timed region."""
+    """Wall time of exactly `steps` back-to-back calls of fn() after `warmup` untimed ones
+    (preceded by WARM_S seconds of the same work), synchronised on both sides, with nothing
+    else enqueued in the timed region."""
+    warm(fn, torch)
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -134,6 +150,7 @@ def region_ms(fn, steps, warmup, torch):
     """Average GPU time per call of fn() over `steps` back-to-back calls: one event pair on
     the current stream around the whole timed region, divided by the call count (per-call
     event pairs would also count the host's submission gaps)."""
+    warm(fn, torch)
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -418,11 +435,12 @@ def dominant_kernel(ctx, steps, torch):
             "valu_issue_share": valu_share, "mfma_busy_frac": mfma_busy}
 
 
-def kernel_only(ctx, steps, torch):
-    """The dominant kernel autotuned (7 candidate shapes x 6 runs), then `steps` times,
-    then a plain device copy of 256 MiB `steps` times (the calibration for the
-    FETCH_SIZE / WRITE_SIZE counters)."""
+def kernel_only(ctx, steps, torch, warm_s):
+    """The dominant kernel autotuned (7 candidate shapes x 6 runs), run for `warm_s` seconds
+    (steady clocks), then `steps` times, then a plain device copy of 256 MiB `steps` times
+    (the calibration for the FETCH_SIZE / WRITE_SIZE counters)."""
     fn = _double_x(ctx, torch)
+    warm(fn, torch, warm_s)
     for _ in range(steps):
         fn()
     src = torch.empty(64 << 20, dtype=torch.float32, device="cuda:0").fill_(1.0)
@@ -434,9 +452,9 @@ def kernel_only(ctx, steps, torch):
     return 0
 
 
-def kernel_only_nns256(ctx, steps, torch):
-    """nnedi3-nns256-win8x6 FP16 MFMA on 8 10-bit 1080p frames: autotuned, then `steps`
-    launches (both passes)."""
+def kernel_only_nns256(ctx, steps, torch, warm_s):
+    """nnedi3-nns256-win8x6 FP16 MFMA on 8 10-bit 1080p frames: autotuned, run for `warm_s`
+    seconds, then `steps` launches (both passes)."""
     import numpy as np
     from mpx.frames import make_plane
     b = 8
@@ -446,6 +464,7 @@ def kernel_only_nns256(ctx, steps, torch):
     pl = ctx.plan("nnedi3-nns256-win8x6", W, H, in_dtype=torch.float16, out_dtype=torch.float16,
                   precision="fp16", max_batch=b)
     r = pl.tune(x, o, iters=2)
+    warm(lambda: pl.run(x, out=o), torch, warm_s)
     for _ in range(steps):
         pl.run(x, out=o)
     torch.cuda.synchronize()
@@ -453,7 +472,7 @@ def kernel_only_nns256(ctx, steps, torch):
     return 0
 
 
-def kernel_only_ravu(ctx, steps, torch):
+def kernel_only_ravu(ctx, steps, torch, warm_s):
     """The RAVU variants of the secondary list, `steps` launches each, at their family's
     default tile shape (the shape the secondary lines' main figures use; a tuner run under
     a profiler would compare distorted timings).  Before each variant's launches a
@@ -471,6 +490,7 @@ def kernel_only_ravu(ctx, steps, torch):
         out = pl.run(xin)
         t = pl.tile()
         tiles.append([t["w"], t["h"]])
+        warm(lambda: pl.run(xin, out=out), torch, warm_s)
         torch.full((1,), float(i), device="cuda:0")   # start marker
         for _ in range(steps):
             pl.run(xin, out=out)
@@ -492,6 +512,9 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant kernel (double_x pass) and a plain copy of known size, for "
                          "hardware-counter runs")
+    ap.add_argument("--kernel-warm", type=float, default=0.1,
+                    help="with --kernel-only: seconds of the same launches before the counted ones "
+                         "(steady clocks for kernel traces; 0 for counter runs)")
     ap.add_argument("--workload", default="headline", choices=("headline", "nns256", "ravu"),
                     help="with --kernel-only: the headline double_x pass (+ copy calibration), "
                          "nnedi3-nns256-win8x6 on 8 10-bit 1080p frames, or the RAVU variants of "
@@ -517,10 +540,10 @@ def main():
 
     if args.kernel_only:
         if args.workload == "nns256":
-            return kernel_only_nns256(ctx, args.steps, torch)
+            return kernel_only_nns256(ctx, args.steps, torch, args.kernel_warm)
         if args.workload == "ravu":
-            return kernel_only_ravu(ctx, args.steps, torch)
-        return kernel_only(ctx, args.steps, torch)
+            return kernel_only_ravu(ctx, args.steps, torch, args.kernel_warm)
+        return kernel_only(ctx, args.steps, torch, args.kernel_warm)
 
     plans = {p: ctx.plan(HEADLINE, W, H, in_dtype=torch.float16, out_dtype=torch.float16, precision=p)
              for p in ("fp16", "fp32")}
