@@ -92,6 +92,48 @@ def test_queue_deadline_and_isolated_failures():
         ctx.close()
 
 
+@pytest.mark.timeout(300)
+def test_queue_concurrent_submitters():
+    """Four host threads submit and wait at the same time (ctypes releases the GIL, so the
+    library's submit, scheduler and completion threads all run concurrently): every
+    request completes with the result it has alone."""
+    import threading
+    import mpx
+    from mpx.queue import Queue
+    names = ("ravu-lite-ar-r3", "ravu-3x-r3", "nnedi3-nns32-win8x4", "ravu-r2-yuv")
+    sizes = ((160, 90), (97, 61), (256, 144), (64, 48), (200, 120))
+    q = Queue(hbm_budget=4 << 30, streams=3, max_batch=4, max_size=(256, 256))
+    results, errors = {}, []
+
+    def worker(t):
+        try:
+            mine = []
+            for i in range(12):
+                v = names[(t + i) % len(names)]
+                w, h = sizes[(3 * t + i) % len(sizes)]
+                x = _frame(v, w, h, 900 + 37 * t + i)
+                prec = "fp16" if v.startswith("nnedi3") else "fp32"
+                mine.append((v, x, prec, q.submit(v, x, precision=prec)))
+            for v, x, prec, tk in mine:
+                results[(t, len(results))] = (v, x, prec, tk.wait(timeout=120))
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    try:
+        ts = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+        for th in ts:
+            th.start()
+        for th in ts:
+            th.join(timeout=240)
+        assert not any(th.is_alive() for th in ts)
+    finally:
+        q.close()
+    assert not errors, errors
+    assert len(results) == 48
+    for v, x, prec, got in results.values():
+        assert torch.equal(got, mpx.upscale(v, x, precision=prec)), (v, tuple(x.shape))
+
+
 def test_submit_refuses_bad_requests():
     from mpx._abi import MpxError
     from mpx.queue import Queue
