@@ -69,3 +69,32 @@ def test_native_on_420_after_cscale_vs_oracle(loc):
     err = float(np.abs(got - ref).max())
     print("ravu-r2-yuv on 4:2:0 (%s siting) vs FP64 oracle: max-abs %.3g" % (loc, err))
     assert err <= 5e-5
+
+
+def test_main_hook_on_yuv_420_converts_to_rgb():
+    """A YUV 4:2:0 source into a MAIN (-rgb) hook: chroma to the luma size, the BT.709
+    limited-range matrix, then the RGB planes through the variant -- the same as running
+    the variant on yuv_to_rgb's planes; the result replaces the frame's planes."""
+    import mpx
+    from mpx.hooks import apply_hook, yuv_to_rgb
+    fr = _yuv(56, 40, seed=21)
+    out, offs, ran = apply_hook("ravu-r2-rgb", fr, (112, 80), bits=10)
+    assert ran and set(out) == {"RGB"} and offs == {"RGB": (-0.5, -0.5)}
+    rgb = yuv_to_rgb(fr, bits=10)
+    assert tuple(rgb.shape) == (3, 40, 56) and rgb.is_cuda
+    assert torch.equal(out["RGB"], mpx.upscale("ravu-r2-rgb", rgb))
+
+
+def test_native_on_integer_420():
+    """Integer 4:2:0 planes into a NATIVE hook: resampled chroma is float, so the luma
+    plane is brought to code / (2^bits - 1) as well; the result equals the same frame given
+    as normalised floats."""
+    from mpx.hooks import apply_hook
+    f = _yuv(48, 32, seed=13)
+    fi = {k: torch.round(v * 1023).to(torch.uint16) for k, v in f.items()}   # 10-bit codes
+    ff = {k: v.to(torch.float32) / 1023 for k, v in fi.items()}
+    a, _, ran_a = apply_hook("ravu-r2-yuv", fi, (96, 64), bits=10)
+    b, _, ran_b = apply_hook("ravu-r2-yuv", ff, (96, 64), bits=10)
+    assert ran_a and ran_b
+    for k in ("Y", "U", "V"):
+        assert torch.allclose(a[k], b[k], atol=1e-6), k
