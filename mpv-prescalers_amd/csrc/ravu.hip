@@ -3,10 +3,10 @@
 // RAVU-3x (compute/ravu-3x-r3.hook:22-163): one thread per input pixel, 3x3 output block
 // (centre = input pixel), 24x3x3 buckets, two LUT texels per tap.
 // RAVU (ravu-r3.hook:15-437): kernel 1 predicts int11 (x+1/2, y+1/2) from the 2r x 2r
-// window into an FP32 scratch plane; kernel 2 predicts int10 (x+1/2, y) and int01
-// (x, y+1/2) on the 45-degree lattice mixing the input and int11, and write the
-// interleaved 2x output (the combine pass, :426-437), one kernel per prediction (each
-// writes one of the two output rows of a pixel).  All read LDS tiles staged with
+// window into an FP32 scratch plane; kernel 2 predicts int10 (x+1/2, y) and then int01
+// (x, y+1/2) on the 45-degree lattice mixing the input and int11, from one staging of
+// both tiles, and writes the interleaved 2x output (the combine pass, :426-437), each
+// prediction with the output row it completes.  All read LDS tiles staged with
 // clamp-to-edge halos, each texture clamped on its own size as mpv's sampler does.
 //
 // Channel forms (ravu-r3-yuv.hook, ravu-r3-rgb.hook): the hash sees plane 0 (yuv) or
@@ -293,16 +293,50 @@ __device__ __host__ constexpr int lat2x(int X, int Y) { return STEP == 2 ? X + Y
 template <int N, int STEP>
 __device__ __host__ constexpr int lat2y(int X, int Y) { return STEP == 2 ? Y - X : Y - X + 1; }
 
-// step 2 (int10, STEP = 2) or step 3 (int01, STEP = 3) on the lattice, and the combine
-// for one output row: STEP 2 writes row 2y = (input, int10) pairs, STEP 3 writes row
-// 2y+1 = (int01, int11) pairs (ravu-r3.hook:426-437), so both kernels store contiguous
-// pairs and each holds one prediction's registers only.
-template <int R, int CH, int STEP, int TX, int TY>
+// One site's step 2 (int10, STEP = 2) or step 3 (int01, STEP = 3) on the lattice, and
+// the combine for one output row: STEP 2 writes row 2y = (input, int10) pairs, STEP 3
+// row 2y+1 = (int01, int11) pairs (ravu-r3.hook:426-437), so each step stores contiguous
+// pairs.  hk / i11: the staged input and int11 tiles, base: the site's LDS offset.
+template <int R, int CH, int STEP, int PW, int PN>
+__device__ __forceinline__ void lattice_site(const Io& io, const __half* __restrict__ lut, const float* hk,
+                                             const float* i11, int base, const FrameTile& ft, int x, int y,
+                                             int64_t plane) {
+  constexpr int P = CH == 0 ? 1 : 3;
+  constexpr int N = 2 * R;
+  auto src = [&](int i) -> const float* {
+    const int X = i / N, Y = i % N;
+    const int px = lat2x<N, STEP>(X, Y), py = lat2y<N, STEP>(X, Y);
+    return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
+  };
+  const Bucket bk = ravu_hash<N, true, RULE_LOG2, true>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
+                                                  [&](int i) { return luma_d<CH>(src(i), 0, PN); });
+  const int row = bucket_row<RULE_LOG2>(bk, 648);
+  if (io.buckets)
+    io.buckets[ft.f * io.bucket_bstride + (STEP - 1) * plane + static_cast<int64_t>(y) * ft.W + x] = row;
+  float res[P];
+  ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
+                  res);
+  const int esz_out = elem_bytes(io.out_dt);
+  char* ob = static_cast<char*>(frame_out(io, ft.f));
+#pragma unroll
+  for (int c = 0; c < P; ++c) {
+    void* o = ob + c * ft.out_cstride * esz_out;
+    if (STEP == 2)
+      store_pair(o, static_cast<int64_t>(2 * y) * ft.out_pitch + 2 * x, io.out_dt, hk[c * PN + base], res[c]);
+    else
+      store_pair(o, static_cast<int64_t>(2 * y + 1) * ft.out_pitch + 2 * x, io.out_dt, res[c], i11[c * PN + base]);
+  }
+}
+
+// Steps 2 and 3 of a tile: both read the same staged input and int11 tiles (lattice
+// offsets -R..R), so one kernel stages them once and runs the two steps one after the
+// other per site; a scheduling barrier between them keeps one step's registers live at a
+// time.
+template <int R, int CH, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void ravu_lattice_kernel(Io io, const __half* __restrict__ lut,
                                                            const float* __restrict__ int11, int64_t int11_frame,
                                                            int total) {
   constexpr int P = CH == 0 ? 1 : 3;
-  constexpr int N = 2 * R;
   constexpr int PW = TX + 2 * R + 1, PH = TY + 2 * R + 1, PN = PW * PH;  // offsets -R..R
   __shared__ float hk[staged_planes<CH>() * PN];   // input (HOOKED)
   __shared__ float i11[staged_planes<CH>() * PN];  // ravu_int11
@@ -328,28 +362,9 @@ __global__ __launch_bounds__(TX * TY) void ravu_lattice_kernel(Io io, const __ha
   const int x = x0 + tx, y = y0 + ty;
   if (x >= ft.W || y >= ft.H) return;
   const int base = (ty + R) * PW + tx + R;  // LDS offset of (x, y)
-  auto src = [&](int i) -> const float* {
-    const int X = i / N, Y = i % N;
-    const int px = lat2x<N, STEP>(X, Y), py = lat2y<N, STEP>(X, Y);
-    return (px & 1) ? i11 + base + ((py - 1) >> 1) * PW + ((px - 1) >> 1) : hk + base + (py >> 1) * PW + (px >> 1);
-  };
-  const Bucket bk = ravu_hash<N, true, RULE_LOG2, true>([&](int i) { return luma_f<CH>(src(i), 0, PN); },
-                                                  [&](int i) { return luma_d<CH>(src(i), 0, PN); });
-  const int row = bucket_row<RULE_LOG2>(bk, 648);
-  if (io.buckets) io.buckets[f * io.bucket_bstride + (STEP - 1) * plane + static_cast<int64_t>(y) * ft.W + x] = row;
-  float res[P];
-  ravu_taps<N, P>(lut + static_cast<int64_t>(row) * RavuLut<N>::NTEX * 4, [&](int c, int i) { return src(i)[c * PN]; },
-                  res);
-  const int esz_out = elem_bytes(io.out_dt);
-  char* ob = static_cast<char*>(frame_out(io, f));
-#pragma unroll
-  for (int c = 0; c < P; ++c) {
-    void* o = ob + c * ft.out_cstride * esz_out;
-    if (STEP == 2)
-      store_pair(o, static_cast<int64_t>(2 * y) * ft.out_pitch + 2 * x, io.out_dt, hk[c * PN + base], res[c]);
-    else
-      store_pair(o, static_cast<int64_t>(2 * y + 1) * ft.out_pitch + 2 * x, io.out_dt, res[c], i11[c * PN + base]);
-  }
+  lattice_site<R, CH, 2, PW, PN>(io, lut, hk, i11, base, ft, x, y, plane);
+  __builtin_amdgcn_sched_barrier(0);
+  lattice_site<R, CH, 3, PW, PN>(io, lut, hk, i11, base, ft, x, y, plane);
 }
 
 template <int T>
@@ -363,11 +378,7 @@ static hipError_t launch_ravu_tile(int r, int channels, Io io, const __half* lut
     hipLaunchKernelGGL((ravu_step1_kernel<R, CH, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame, total); \
     hipError_t e = hipGetLastError();                                                                           \
     if (e != hipSuccess) return e;                                                                              \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 2, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame,    \
-                       total);                                                                                  \
-    e = hipGetLastError();                                                                                      \
-    if (e != hipSuccess) return e;                                                                              \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, 3, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame,    \
+    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame,       \
                        total);                                                                                  \
     return hipGetLastError();                                                                                   \
   }
