@@ -35,7 +35,7 @@ struct TileShape {
 };
 constexpr TileShape kLiteTiles[] = {{64, 4}, {32, 8}, {32, 4}};
 constexpr TileShape k3xTiles[] = {{32, 8}, {64, 4}};
-constexpr TileShape kRavuTiles[] = {{32, 16}, {64, 8}, {32, 8}};
+constexpr TileShape kRavuTiles[] = {{32, 8}, {32, 16}, {64, 8}};
 constexpr TileShape kZoomTiles[] = {{32, 8}, {16, 16}};
 constexpr int kLiteTileCount = sizeof(kLiteTiles) / sizeof(kLiteTiles[0]);
 constexpr int k3xTileCount = sizeof(k3xTiles) / sizeof(k3xTiles[0]);

@@ -20,8 +20,9 @@ namespace mpx {
 
 // Tiles: TX x TY pixels per block of TX * TY threads, one input pixel (3x) or site (full
 // RAVU) per thread, from the families' shape lists (k3xTiles, kRavuTiles).  Full RAVU's halo
-// is up to 9 rows, so its default tile is 32 x 16 on 512 threads: r4 lattice 41 x 25
-// staged samples per 512 sites, against 73 x 13 per 256 for a 64 x 4 tile.
+// is up to 9 rows, so its tiles are taller than 3x's: 32 x 8 by default (r4 lattice: 41 x 17
+// staged samples per 256 sites, against 73 x 13 for a 64 x 4 tile), 32 x 16 and 64 x 8 on
+// 512 threads as alternatives.
 
 
 // Stage P planes of a frame into LDS: tile[c][r][col] = plane c at
