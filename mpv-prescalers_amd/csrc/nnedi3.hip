@@ -458,6 +458,27 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Cross-lane sums in VGPRs (v_permlane{16,32}_swap, no LDS round trip), two values at a
+// time, each result scattered to the lanes that own it.  pair32_sum: lanes 0-31 get
+// a[l] + a[l ^ 32], lanes 32-63 b[l] + b[l ^ 32].  pair16_sum, by rows of 16 lanes: rows 0
+// and 2 get a[l] + a[l ^ 16], rows 1 and 3 b[l] + b[l ^ 16].  (The swap exchanges the upper
+// half / odd rows of its first operand with the lower half / even rows of its second.)
+// Each sum adds the same two operands as v + shfl_xor(v, 32 / 16) does.
+__device__ __forceinline__ float pair32_sum(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float pair16_sum(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// Four values' sums over the lanes l, l^16, l^32, l^48, scattered: lanes of row q (16q ..
+// 16q+15) get the sum of p[q], added as (p(l) + p(l^16)) + (p(l^32) + p(l^48)) like two
+// shfl_xor steps (16, then 32).
+__device__ __forceinline__ float quad_sum_scatter(const float p[4]) {
+  return pair32_sum(pair16_sum(p[0], p[1]), pair16_sum(p[2], p[3]));
+}
+
 // Block: 4 waves; a staged tile of NX x NYS = 64 x 8 pixels is computed as two sub-tiles
 // of NX x NY = 64 x 4, wave w owning pixel row w of each (pixel groups 4w..4w+3 of 16):
 // one halo load and one block barrier serve 512 pixels (win8x6 forms: 4-6 % faster than
@@ -545,8 +566,9 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     d = d01 * d23;
   };
 
-  // softmax-weighted sums of one pixel group over this wave's neuron tiles; on return
-  // every lane holds the sums of pixel (g*16 + row)
+  // softmax-weighted sums of one pixel group over this wave's neuron tiles: this lane's
+  // part (4 neurons per tile) of the sums of pixel g*16 + row; lanes l, l^16, l^32, l^48
+  // hold the parts of the same pixel
   auto group_sums = [&](int g, float& ws, float& vs) {
     typename M::B bx;
     M::load_b(bx, xhi, xlo, g * 16 + row, quad);  // B column of this lane: pixel g*16 + row
@@ -569,11 +591,6 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
       if (t % 2 == 1 || t == TPW - 1) vs = fmaf(n, __builtin_amdgcn_rcpf(d), vs);
     }
     if constexpr (SCALED) vs *= inv_acc;
-    // lanes l, l^16, l^32, l^48 hold the same pixel
-    ws += __shfl_xor(ws, 16);
-    vs += __shfl_xor(vs, 16);
-    ws += __shfl_xor(ws, 32);
-    vs += __shfl_xor(vs, 32);
   };
 
   NTile<WH, AXIS, NYS> ht;
@@ -633,28 +650,26 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
 
       float wsum = 0.f, vsum = 0.f;
       if constexpr (NSL == 1) {
-        // this wave's own rows: groups 4*wave + j; after group j, the lanes of quad j hold
-        // their own pixel's sums
+        // this wave's own rows: groups 4*wave + j; the lanes of quad j own pixel
+        // (4*wave + j)*16 + row, their thread's pixel
         wave_lds_sync();
+        float ws[4], vs[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          float ws, vs;
-          group_sums(wave * 4 + j, ws, vs);
-          if (quad == j) {
-            wsum = ws;
-            vsum = vs;
-          }
-        }
+        for (int j = 0; j < 4; ++j) group_sums(wave * 4 + j, ws[j], vs[j]);
+        wsum = quad_sum_scatter(ws);
+        vsum = quad_sum_scatter(vs);
       } else {
         // (also orders the previous sub-tile's reads of `part` before this one's writes)
         __syncthreads();
-        for (int g = pslice; g < NGROUP; g += PSL) {
-          float ws, vs;
-          group_sums(g, ws, vs);
-          if (quad == 0) {
-            part[nslice][g * 16 + row][0] = ws;
-            part[nslice][g * 16 + row][1] = vs;
-          }
+        static_assert((NGROUP / PSL) % 4 == 0, "groups reduced four at a time");
+        for (int g = pslice; g < NGROUP; g += 4 * PSL) {
+          float ws[4], vs[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) group_sums(g + j * PSL, ws[j], vs[j]);
+          // the lanes of quad j hold the sums of group g + j*PSL
+          const int gq = g + quad * PSL;
+          part[nslice][gq * 16 + row][0] = quad_sum_scatter(ws);
+          part[nslice][gq * 16 + row][1] = quad_sum_scatter(vs);
         }
         __syncthreads();
 #pragma unroll
@@ -813,8 +828,8 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
     n = fmaf(n01, d23, n23 * d01);
     d = d01 * d23;
   };
-  // softmax-weighted sums of pixel group g over this wave's neuron tiles; on return lanes
-  // l and l^32 hold the sums of pixel g*32 + l%32
+  // softmax-weighted sums of pixel group g over this wave's neuron tiles: this lane's part
+  // (16 neurons per tile) of the sums of pixel g*32 + l%32, the other part in lane l^32
   auto group_sums = [&](int g, float& ws, float& vs) {
     typename M::B bx;
     M::load_b(bx, xhi, xlo, g * 32 + col, half);
@@ -833,8 +848,6 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
       }
     }
     if constexpr (SCALED) vs *= inv_acc;
-    ws += __shfl_xor(ws, 32);
-    vs += __shfl_xor(vs, 32);
   };
 
   NTile<WH, AXIS, NYS> ht;
@@ -888,26 +901,42 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
 
       float wsum = 0.f, vsum = 0.f;
       if constexpr (NSL == 1) {
+        // groups 2*wave + j; the lanes of half j own pixel (2*wave + j)*32 + col, their
+        // thread's pixel.  The three-product forms sit at the register limit, where the
+        // swap's second result register makes them spill more (BF16 nns64 +10 %): they
+        // keep the LDS shuffle, one group at a time.
         wave_lds_sync();
+        if constexpr (TERMS == 3) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          float ws, vs;
-          group_sums(wave * 2 + j, ws, vs);
-          if (half == j) {
-            wsum = ws;
-            vsum = vs;
+          for (int j = 0; j < 2; ++j) {
+            float ws, vs;
+            group_sums(wave * 2 + j, ws, vs);
+            ws += __shfl_xor(ws, 32);
+            vs += __shfl_xor(vs, 32);
+            if (half == j) {
+              wsum = ws;
+              vsum = vs;
+            }
           }
+        } else {
+          float ws[2], vs[2];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) group_sums(wave * 2 + j, ws[j], vs[j]);
+          wsum = pair32_sum(ws[0], ws[1]);
+          vsum = pair32_sum(vs[0], vs[1]);
         }
       } else {
         // (also orders the previous sub-tile's reads of `part` before this one's writes)
         __syncthreads();
-        for (int g = pslice; g < NGROUP; g += PSL) {
-          float ws, vs;
-          group_sums(g, ws, vs);
-          if (half == 0) {
-            part[nslice][g * 32 + col][0] = ws;
-            part[nslice][g * 32 + col][1] = vs;
-          }
+        static_assert((NGROUP / PSL) % 2 == 0, "groups reduced two at a time");
+        for (int g = pslice; g < NGROUP; g += 2 * PSL) {
+          float ws[2], vs[2];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) group_sums(g + j * PSL, ws[j], vs[j]);
+          // the lanes of half j hold the sums of group g + j*PSL
+          const int gh = g + half * PSL;
+          part[nslice][gh * 32 + col][0] = pair32_sum(ws[0], ws[1]);
+          part[nslice][gh * 32 + col][1] = pair32_sum(vs[0], vs[1]);
         }
         __syncthreads();
 #pragma unroll
