@@ -3,6 +3,7 @@
 
 #include <atomic>
 
+#include <hip/hip_ext.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
@@ -23,6 +24,19 @@ inline int device_cu_count() {
     cache[dev].store(n, std::memory_order_relaxed);
   }
   return n;
+}
+
+// Launch with an optional completion event carried by the kernel's own dispatch
+// (hipExtLaunchKernel's stop event) instead of a separate event record after it: the
+// stream's next work then follows this kernel with no marker packet in between.  (The
+// event completes when this kernel has, hence after all earlier work of the stream.)
+template <typename K, typename... A>
+inline hipError_t launch_done(K kern, dim3 grid, dim3 block, hipStream_t st, hipEvent_t done, A... args) {
+  if (done)
+    hipExtLaunchKernelGGL(kern, grid, block, 0, st, nullptr, done, 0, args...);
+  else
+    hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
+  return hipGetLastError();
 }
 
 // Tile shapes each RAVU family's kernels are built for: one block per tile, one thread per

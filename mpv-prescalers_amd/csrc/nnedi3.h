@@ -36,8 +36,9 @@ struct NnediWeights {
 
 int nnedi3_prepare(mpx_ctx* ctx, int nns, int wh, int precision, NnediWeights* out);
 
-// double_y (if dy) then double_x (if dx); `tmp` holds the W x 2H double_y output when both.
+// double_y (if dy) then double_x (if dx); `tmp` holds the W x 2H double_y output when both;
+// done (optional): an event the last pass's dispatch completes (launch_done).
 hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy, float* tmp, int64_t tmp_frame,
-                         hipStream_t st);
+                         hipStream_t st, hipEvent_t done = nullptr);
 
 }  // namespace mpx

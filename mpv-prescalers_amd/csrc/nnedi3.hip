@@ -959,7 +959,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
 // ----------------------------------------------------------------------------- launcher
 
 template <int WH, int AXIS>
-static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
+static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st, hipEvent_t done) {
   // tiles of NX x ty pixels (ty = NY for the plain kernel, NYS for the persistent MFMA
   // kernel); fills the per-frame prefix of sized launches
   auto count_tiles = [&](int ty) {
@@ -969,11 +969,10 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
     return p.tiles[p.ntab_src];
   };
   int total = count_tiles(nw.precision == 0 ? NY : NYS);
-  if (nw.precision == 0) {
-    hipLaunchKernelGGL((nnedi3_valu<WH, AXIS>), dim3(total), dim3(256), 0, st, p, static_cast<const float*>(nw.w),
+  if (nw.precision == 0)
+    return launch_done(nnedi3_valu<WH, AXIS>, dim3(total), dim3(256), st, done, p, static_cast<const float*>(nw.w),
                        nw.nns, total);
-    return hipGetLastError();
-  }
+  hipError_t le = hipErrorInvalidValue;  // the MFMA launch's status
   const int nt = nw.nns / 16;
   const float inv_acc = 1.0f / (nw.xscale * nw.wscale);
   // Neuron tiles are split over NSL waves only when one wave cannot hold them all
@@ -993,8 +992,8 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
     }                                                                                                 \
     const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
     const int grid = min(total, device_cu_count() * k);                                                      \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
-                       inv_acc, total);                                                               \
+    le = launch_done(kern, dim3(grid), dim3(256), st, done, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
+                     inv_acc, total);                                                                 \
   }
 #define MPX_NN_NT(FM, TERMS, SC)                    \
   switch (nt) {                                     \
@@ -1028,7 +1027,7 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
     }                                                                                                 \
     const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
     const int grid = min(total, device_cu_count() * k);                                               \
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, p, nw.w, nw.w_lo, nw.bias, nw.xscale, inv_acc, total); \
+    le = launch_done(kern, dim3(grid), dim3(256), st, done, p, nw.w, nw.w_lo, nw.bias, nw.xscale, inv_acc, total); \
   }
 #define MPX_NN32_NT(FM, TERMS, SC)                  \
   switch (nw.nns / 32) {                            \
@@ -1055,7 +1054,7 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
       } else {
         MPX_NN32_NT(F_BF16, 3, false)
       }
-      return hipGetLastError();
+      return le;
     }
   }
 #undef MPX_NN32_NT
@@ -1081,11 +1080,11 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st) {
   }
 #undef MPX_NN_NT
 #undef MPX_NN
-  return hipGetLastError();
+  return le;
 }
 
 hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy, float* tmp, int64_t tmp_frame,
-                         hipStream_t st) {
+                         hipStream_t st, hipEvent_t done) {
   hipError_t e = hipSuccess;
   if (dy) {
     NPass p{};
@@ -1128,7 +1127,9 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
         p.tab_dst[f] = dx ? static_cast<void*>(tmp + f * tmp_frame) : io.tab_out[f];
       }
     }
-    e = nw.wh == 4 ? run_pass<4, 0>(nw, p, st) : run_pass<6, 0>(nw, p, st);
+    // (the completion event goes with the last pass)
+    hipEvent_t dn = dx ? nullptr : done;
+    e = nw.wh == 4 ? run_pass<4, 0>(nw, p, st, dn) : run_pass<6, 0>(nw, p, st, dn);
     if (e != hipSuccess) return e;
   }
   if (dx) {
@@ -1175,7 +1176,7 @@ hipError_t launch_nnedi3(const NnediWeights& nw, const Io& io, bool dx, bool dy,
         p.tab_src[f] = dy ? static_cast<const void*>(tmp + f * tmp_frame) : io.tab_in[f];
       }
     }
-    e = nw.wh == 4 ? run_pass<4, 1>(nw, p, st) : run_pass<6, 1>(nw, p, st);
+    e = nw.wh == 4 ? run_pass<4, 1>(nw, p, st, done) : run_pass<6, 1>(nw, p, st, done);
   }
   return e;
 }

@@ -35,9 +35,10 @@ void zoom_plan_destroy(mpx_ctx* ctx, mpx_plan* plan);
 hipError_t launch_ravu_3x(int r, int channels, const Io& io, const __half* lut, int tile, hipStream_t st);
 
 // RAVU full: int11 into `int11` scratch (frame stride int11_frame floats), then int10/int01
-// and the interleaved 2x output.  lut = FP16 [648][n^2/8][4]; tile: index into kRavuTiles.
+// and the interleaved 2x output.  lut = FP16 [648][n^2/8][4]; tile: index into kRavuTiles;
+// done (optional): an event the last kernel's dispatch completes (launch_done).
 hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
-                       int tile, hipStream_t st);
+                       int tile, hipStream_t st, hipEvent_t done = nullptr);
 
 // RAVU-zoom(-ar) to out_w x out_h; tile: index into kZoomTiles (output pixels per block).
 hipError_t launch_ravu_zoom(int r, bool ar, int channels, const Io& io, const __half* lut, const __half* lut_ar,

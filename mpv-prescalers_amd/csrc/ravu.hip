@@ -370,7 +370,7 @@ __global__ __launch_bounds__(TX * TY) void ravu_lattice_kernel(Io io, const __ha
 
 template <int T>
 static hipError_t launch_ravu_tile(int r, int channels, Io io, const __half* lut, float* int11, int64_t int11_frame,
-                                   hipStream_t st) {
+                                   hipStream_t st, hipEvent_t done) {
   constexpr int TX = kRavuTiles[T].w, TY = kRavuTiles[T].h;
   const int total = launch_tiles<TX, TY>(io);
   const dim3 grid(total), block(TX * TY);
@@ -379,9 +379,8 @@ static hipError_t launch_ravu_tile(int r, int channels, Io io, const __half* lut
     hipLaunchKernelGGL((ravu_step1_kernel<R, CH, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame, total); \
     hipError_t e = hipGetLastError();                                                                           \
     if (e != hipSuccess) return e;                                                                              \
-    hipLaunchKernelGGL((ravu_lattice_kernel<R, CH, TX, TY>), grid, block, 0, st, io, lut, int11, int11_frame,       \
+    return launch_done(ravu_lattice_kernel<R, CH, TX, TY>, grid, block, st, done, io, lut, int11, int11_frame,      \
                        total);                                                                                  \
-    return hipGetLastError();                                                                                   \
   }
   MPX_RAVU(2, 0) MPX_RAVU(2, 1) MPX_RAVU(2, 2) MPX_RAVU(3, 0) MPX_RAVU(3, 1) MPX_RAVU(3, 2) MPX_RAVU(4, 0)
   MPX_RAVU(4, 1) MPX_RAVU(4, 2)
@@ -390,12 +389,12 @@ static hipError_t launch_ravu_tile(int r, int channels, Io io, const __half* lut
 }
 
 hipError_t launch_ravu(int r, int channels, const Io& io, const __half* lut, float* int11, int64_t int11_frame,
-                       int tile, hipStream_t st) {
+                       int tile, hipStream_t st, hipEvent_t done) {
   static_assert(kRavuTileCount == 3, "one case per tile shape");
   switch (tile) {
-    case 0: return launch_ravu_tile<0>(r, channels, io, lut, int11, int11_frame, st);
-    case 1: return launch_ravu_tile<1>(r, channels, io, lut, int11, int11_frame, st);
-    case 2: return launch_ravu_tile<2>(r, channels, io, lut, int11, int11_frame, st);
+    case 0: return launch_ravu_tile<0>(r, channels, io, lut, int11, int11_frame, st, done);
+    case 1: return launch_ravu_tile<1>(r, channels, io, lut, int11, int11_frame, st, done);
+    case 2: return launch_ravu_tile<2>(r, channels, io, lut, int11, int11_frame, st, done);
     default: return hipErrorInvalidValue;
   }
 }

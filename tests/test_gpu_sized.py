@@ -103,6 +103,44 @@ def test_plan_refuses_another_stream_while_its_intermediates_are_busy():
     ctx.close()
 
 
+def test_ravu_plan_refuses_another_stream_while_its_int11_plane_is_busy():
+    """Full RAVU keeps its int11 plane in the plan's scratch, and its last kernel (steps 2
+    and 3) completes the plan's event.  With slow FP32 NNEDI3 work queued ahead of it on
+    stream A, the RAVU run on A is still in flight when the same plan is called on stream B:
+    refused, nothing enqueued; after A finishes, B runs and equals A's result."""
+    import mpx
+    from mpx.frames import make_frame, make_plane
+    w, h = 1920, 1080
+    ctx = mpx.Context(0)
+    slow = ctx.plan("nnedi3-nns256-win8x6", w, h, in_dtype=torch.float16, out_dtype=torch.float32,
+                    precision="fp32", max_batch=2)
+    plan = ctx.plan("ravu-r4-yuv", w, h, in_dtype=torch.float16, out_dtype=torch.float32, max_batch=2)
+    xs = torch.from_numpy(np.stack([make_plane(w, h, seed=80 + i, bits=10) for i in range(2)])
+                          .astype(np.float16)).cuda()
+    os_ = torch.empty((2, 2 * h, 2 * w), device="cuda")
+    x = torch.from_numpy(np.stack([make_frame(w, h, 90 + i, bits=10, channels=3) for i in range(2)])
+                         .astype(np.float16)).cuda()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    oa = torch.empty((2, 3, 2 * h, 2 * w), device="cuda")
+    ob = torch.full((1, 3, 2 * h, 2 * w), -1.0, device="cuda")
+    torch.cuda.synchronize()
+    print("FP32 nns256 then ravu-r4-yuv on stream A, then the RAVU plan on stream B", flush=True)
+    slow.run(xs, out=os_, stream=sa)         # ~8 ms of work ahead of the RAVU run on A
+    plan.run(x, out=oa, stream=sa)
+    with pytest.raises(mpx.MpxError) as ei:
+        plan.run(x[:1], out=ob, stream=sb)
+    assert ei.value.code == -1 and "another stream" in str(ei.value)
+    sa.synchronize()
+    sb.synchronize()
+    assert bool((ob == -1.0).all())           # the refused run enqueued nothing
+    plan.run(x[:1], out=ob, stream=sb)        # A is done: B may take the plan
+    sb.synchronize()
+    assert torch.equal(ob[0], oa[0])
+    plan.close()
+    slow.close()
+    ctx.close()
+
+
 def test_frame_span_limits_refused_before_launch():
     """The staging loaders address samples with 32-bit element offsets: mpx_run refuses a
     pitch of 2^24 elements or an input frame spanning more than 2^32 elements with
