@@ -38,7 +38,11 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 
 constexpr int NX = 64;  // predicted pixels per block tile: NX x NY
 constexpr int NY = 4;
-constexpr int NYS = 8;  // rows of the persistent MFMA kernel's staged tile: two NY sub-tiles
+// Rows of the persistent MFMA kernels' staged tile (template argument TY): 8 = two NY
+// sub-tiles per halo load and block barrier; the one-product FP16 forms also have 16-row
+// instances (four sub-tiles), launched where they keep the occupancy and give every block
+// at least two tiles (run_pass).
+constexpr int NYS = 8;
 
 struct NPass {
   const void* src;
@@ -479,10 +483,10 @@ __device__ __forceinline__ float quad_sum_scatter(const float p[4]) {
   return pair32_sum(pair16_sum(p[0], p[1]), pair16_sum(p[2], p[3]));
 }
 
-// Block: 4 waves; a staged tile of NX x NYS = 64 x 8 pixels is computed as two sub-tiles
-// of NX x NY = 64 x 4, wave w owning pixel row w of each (pixel groups 4w..4w+3 of 16):
-// one halo load and one block barrier serve 512 pixels (win8x6 forms: 4-6 % faster than
-// 64 x 4 tiles).  The NT = nns/16 neuron tiles are split into NSL slices; TPW = NT/NSL
+// Block: 4 waves; a staged tile of NX x TY = 64 x 8 (or 64 x 16) pixels is computed as
+// TY / NY sub-tiles of NX x NY = 64 x 4, wave w owning pixel row w of each (pixel groups
+// 4w..4w+3 of 16): one halo load and one block barrier serve 512 (1024) pixels (win8x6
+// forms: 4-6 % faster than 64 x 4 tiles; 64 x 16: headline -3 %, nns64 -5 to -10 %).  The NT = nns/16 neuron tiles are split into NSL slices; TPW = NT/NSL
 // neuron tiles per wave, held in VGPRs for the whole launch.
 //  * NSL = 1 (nns <= 64): every wave runs all neurons for its own 64 pixels, so the
 //    normalised rows it writes to LDS are read back by the same wave and the softmax sums
@@ -496,12 +500,12 @@ __device__ __forceinline__ float quad_sum_scatter(const float p[4]) {
 // parts out of the subnormal range (MFMA flushes FP16 subnormal inputs).
 // The halo tile is double-buffered: the block's next tile is fetched into registers while
 // this one computes and lands in the other buffer.
-template <int WH, int AXIS, int FORM, int TERMS, int NSL, int TPW, bool SCALED>
+template <int WH, int AXIS, int FORM, int TERMS, int NSL, int TPW, bool SCALED, int TY = NYS>
 __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __restrict__ w_hi,
                                                       const void* __restrict__ w_lo, const int* __restrict__ wsc,
                                                       const float* __restrict__ bias, float xscale, float inv_acc,
                                                       int total_tiles) {
-  using G = Geo<WH, AXIS, NYS>;
+  using G = Geo<WH, AXIS, TY>;
   constexpr int K = G::K;
   using M = Mma<FORM, K, TERMS>;
   using T = typename M::T;
@@ -509,7 +513,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   constexpr int NPX = NX * NY;            // 256 pixels per tile
   constexpr int NGROUP = NPX / 16;        // 16 pixel groups
   constexpr int PSL = 4 / NSL;            // pixel slices
-  using LS = LineStats<WH, AXIS, NYS>;
+  using LS = LineStats<WH, AXIS, TY>;
   // the one-product FP16 forms on 8x6 windows take the window statistics from shared line
   // statistics (their variance is good to a few ulp; the split forms, which resolve ~22 bits,
   // keep the per-pixel two-pass variance, good to one).  On 8x4 windows the table's build
@@ -593,9 +597,9 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
     if constexpr (SCALED) vs *= inv_acc;
   };
 
-  NTile<WH, AXIS, NYS> ht;
+  NTile<WH, AXIS, TY> ht;
   auto issue = [&](int id) {
-    const PassTile q = pass_tile<NYS>(p, id);
+    const PassTile q = pass_tile<TY>(p, id);
     ht.issue(frame_src(p, q.f), p.src_dt, q.sp, 0, q.W, q.H, q.x0 + G::OX, q.y0 + G::OY);
   };
   // launch-order index L = blockIdx + i*gridDim walks [0, total); tile = xcd_tile(L)
@@ -603,7 +607,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   int buf = 0;
   for (int L = blockIdx.x; L < total_tiles; L += gridDim.x, buf ^= 1) {
     const int tileid = xcd_tile(L, total_tiles);
-    const PassTile pt = pass_tile<NYS>(p, tileid);
+    const PassTile pt = pass_tile<TY>(p, tileid);
     const int f = pt.f;
     const int y0 = pt.y0, x0 = pt.x0;
     // The other buffer was last read two tiles ago, before the barrier below was last
@@ -617,10 +621,10 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
       __syncthreads();
     }
 
-    // the staged tile holds NYS = 2 * NY rows: two sub-tiles of NX x NY pixels, one
-    // pixel per thread each, sharing one halo load and one block barrier
+    // the staged tile holds TY rows: TY / NY sub-tiles of NX x NY pixels, one pixel per
+    // thread each, sharing one halo load and one block barrier
 #pragma unroll 1
-    for (int sub = 0; sub < NYS / NY; ++sub) {
+    for (int sub = 0; sub < TY / NY; ++sub) {
       // one thread per pixel: normalised window -> 16-bit rows (hi, lo), zero padded to KP
       const int tx = threadIdx.x % NX, ty = threadIdx.x / NX + sub * NY;
       float m, sd;
@@ -768,11 +772,11 @@ struct Mma32 {
 // wave (weights and biases in VGPRs for the launch).  NSL = 1: wave w owns groups 2w and
 // 2w+1, its own pixel row, so its LDS rows and sums stay within the wave; NSL > 1: waves
 // of one slice share the groups, partial sums meet in LDS.
-template <int WH, int AXIS, int FORM, int TERMS, int NSL, int TPW, bool SCALED>
+template <int WH, int AXIS, int FORM, int TERMS, int NSL, int TPW, bool SCALED, int TY = NYS>
 __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __restrict__ w_hi,
                                                         const void* __restrict__ w_lo, const float* __restrict__ bias,
                                                         float xscale, float inv_acc, int total_tiles) {
-  using G = Geo<WH, AXIS, NYS>;
+  using G = Geo<WH, AXIS, TY>;
   constexpr int K = G::K;
   using M = Mma32<FORM, K, TERMS>;
   using T = typename M::T;
@@ -781,7 +785,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
   constexpr int NGROUP = NPX / 32;  // 8 pixel groups
   constexpr int PSL = 4 / NSL;      // pixel slices
   static_assert(NSL * PSL == 4, "4 waves per block");
-  using LS = LineStats<WH, AXIS, NYS>;
+  using LS = LineStats<WH, AXIS, TY>;
   // the one-product FP16 forms on 8x6 windows take the window statistics from shared line
   // statistics (their variance is good to a few ulp; the split forms, which resolve ~22 bits,
   // keep the per-pixel two-pass variance, good to one).  On 8x4 windows the table's build
@@ -850,15 +854,15 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
     if constexpr (SCALED) vs *= inv_acc;
   };
 
-  NTile<WH, AXIS, NYS> ht;
+  NTile<WH, AXIS, TY> ht;
   auto issue = [&](int id) {
-    const PassTile q = pass_tile<NYS>(p, id);
+    const PassTile q = pass_tile<TY>(p, id);
     ht.issue(frame_src(p, q.f), p.src_dt, q.sp, 0, q.W, q.H, q.x0 + G::OX, q.y0 + G::OY);
   };
   if (static_cast<int>(blockIdx.x) < total_tiles) issue(xcd_tile(blockIdx.x, total_tiles));
   int buf = 0;
   for (int L = blockIdx.x; L < total_tiles; L += gridDim.x, buf ^= 1) {
-    const PassTile pt = pass_tile<NYS>(p, xcd_tile(L, total_tiles));
+    const PassTile pt = pass_tile<TY>(p, xcd_tile(L, total_tiles));
     const int f = pt.f;
     const int y0 = pt.y0, x0 = pt.x0;
     // the other buffer was last read two tiles ago, before the barrier below was last
@@ -873,7 +877,7 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
     }
 
 #pragma unroll 1
-    for (int sub = 0; sub < NYS / NY; ++sub) {
+    for (int sub = 0; sub < TY / NY; ++sub) {
       const int tx = threadIdx.x % NX, ty = threadIdx.x / NX + sub * NY;
       float m, sd, orig;
       {
@@ -960,8 +964,8 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma32(NPass p, const void* __r
 
 template <int WH, int AXIS>
 static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st, hipEvent_t done) {
-  // tiles of NX x ty pixels (ty = NY for the plain kernel, NYS for the persistent MFMA
-  // kernel); fills the per-frame prefix of sized launches
+  // tiles of NX x ty pixels (ty = NY for the plain kernel, the staged rows TY for the
+  // persistent MFMA kernels); fills the per-frame prefix of sized launches
   auto count_tiles = [&](int ty) {
     if (!p.sized) return ((p.W + NX - 1) / NX) * ((p.H + ty - 1) / ty) * p.batch;
     p.tiles[0] = 0;
@@ -977,23 +981,44 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st, hip
   const float inv_acc = 1.0f / (nw.xscale * nw.wscale);
   // Neuron tiles are split over NSL waves only when one wave cannot hold them all
   // (nns >= 128); otherwise every wave owns all neurons for its own pixel groups.
-  // Persistent grid: as many blocks as the occupancy of the instance allows per CU.
+  // Persistent grid: as many blocks as the occupancy of the instance allows per CU (or the
+  // tuned blocks per CU below it).
+  auto launch_inst = [&](auto kern, int per_cu, int ty, auto... args) {
+    total = count_tiles(ty);
+    const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;
+    const int grid = min(total, device_cu_count() * k);
+    return launch_done(kern, dim3(grid), dim3(256), st, done, p, args..., total);
+  };
+  // The one-product FP16 forms also have 16-row staged tiles: taken where the instance keeps
+  // its occupancy (LDS, VGPRs) and the grid's blocks still get 1.5 tiles each or more (a
+  // 1080p frame's double_y pass: 2040 tiles for 1024 blocks; small frames keep 8 rows,
+  // which spreads them over more blocks).
+  auto tall_pays = [&](int occ8, int occ16) {
+    const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, occ16) : occ16;
+    return occ16 >= occ8 && 2 * count_tiles(16) >= 3 * device_cu_count() * k;
+  };
+  // Occupancy of an instance, queried once (no runtime queries while capturing); callers
+  // may be on several threads: an atomic cache (a race only repeats the same query).
+#define MPX_OCC(var, kern)                                                                           \
+  static std::atomic<int> var##_cache{0};                                                            \
+  int var = var##_cache.load(std::memory_order_relaxed);                                             \
+  if (!var) {                                                                                        \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&var, kern, 256, 0) != hipSuccess || var < 1)   \
+      var = 1;                                                                                       \
+    var##_cache.store(var, std::memory_order_relaxed);                                               \
+  }
 #define MPX_NN(FM, TERMS, NSL, TPW, SC)                                                               \
   {                                                                                                   \
-    auto kern = nnedi3_mfma<WH, AXIS, FM, TERMS, NSL, TPW, SC>;                                       \
-    /* per instance, queried once (no runtime queries while capturing); callers may be on */        \
-    /* several threads: an atomic cache (a race only repeats the same query)                */        \
-    static std::atomic<int> occ{0};                                                                   \
-    int per_cu = occ.load(std::memory_order_relaxed);                                                 \
-    if (!per_cu) {                                                                                    \
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) \
-        per_cu = 1;                                                                                   \
-      occ.store(per_cu, std::memory_order_relaxed);                                                   \
+    auto k8 = nnedi3_mfma<WH, AXIS, FM, TERMS, NSL, TPW, SC, 8>;                                      \
+    MPX_OCC(o8, k8)                                                                                   \
+    bool tall = false;                                                                                \
+    if constexpr (FM == F_F16 && TERMS == 1) {                                                        \
+      auto k16 = nnedi3_mfma<WH, AXIS, FM, TERMS, NSL, TPW, SC, 16>;                                  \
+      MPX_OCC(o16, k16)                                                                               \
+      if ((tall = tall_pays(o8, o16)))                                                                \
+        le = launch_inst(k16, o16, 16, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, inv_acc);          \
     }                                                                                                 \
-    const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
-    const int grid = min(total, device_cu_count() * k);                                                      \
-    le = launch_done(kern, dim3(grid), dim3(256), st, done, p, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, \
-                     inv_acc, total);                                                                 \
+    if (!tall) le = launch_inst(k8, o8, 8, nw.w, nw.w_lo, nw.wexp, nw.bias, nw.xscale, inv_acc);      \
   }
 #define MPX_NN_NT(FM, TERMS, SC)                    \
   switch (nt) {                                     \
@@ -1017,17 +1042,15 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st, hip
   // the 32x32 form measured 3-9 % slower)
 #define MPX_NN32(FM, TERMS, NSL, TPW, SC)                                                             \
   {                                                                                                   \
-    auto kern = nnedi3_mfma32<WH, AXIS, FM, TERMS, NSL, TPW, SC>;                                     \
-    static std::atomic<int> occ{0};                                                                   \
-    int per_cu = occ.load(std::memory_order_relaxed);                                                 \
-    if (!per_cu) {                                                                                    \
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) \
-        per_cu = 1;                                                                                   \
-      occ.store(per_cu, std::memory_order_relaxed);                                                   \
+    auto k8 = nnedi3_mfma32<WH, AXIS, FM, TERMS, NSL, TPW, SC, 8>;                                    \
+    MPX_OCC(o8, k8)                                                                                   \
+    bool tall = false;                                                                                \
+    if constexpr (FM == F_F16 && TERMS == 1) {                                                        \
+      auto k16 = nnedi3_mfma32<WH, AXIS, FM, TERMS, NSL, TPW, SC, 16>;                                \
+      MPX_OCC(o16, k16)                                                                               \
+      if ((tall = tall_pays(o8, o16))) le = launch_inst(k16, o16, 16, nw.w, nw.w_lo, nw.bias, nw.xscale, inv_acc); \
     }                                                                                                 \
-    const int k = nw.grid_per_cu > 0 ? min(nw.grid_per_cu, per_cu) : per_cu;                          \
-    const int grid = min(total, device_cu_count() * k);                                               \
-    le = launch_done(kern, dim3(grid), dim3(256), st, done, p, nw.w, nw.w_lo, nw.bias, nw.xscale, inv_acc, total); \
+    if (!tall) le = launch_inst(k8, o8, 8, nw.w, nw.w_lo, nw.bias, nw.xscale, inv_acc);               \
   }
 #define MPX_NN32_NT(FM, TERMS, SC)                  \
   switch (nw.nns / 32) {                            \
@@ -1080,6 +1103,7 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st, hip
   }
 #undef MPX_NN_NT
 #undef MPX_NN
+#undef MPX_OCC
   return le;
 }
 

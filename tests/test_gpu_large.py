@@ -43,3 +43,22 @@ def test_large_frame_equals_crop(name, w, h, prec, crop):
     b = small[..., k * M:k * (ch - M), k * M:k * (cw - M)]
     assert a.shape == b.shape
     assert torch.equal(a, b), (name, (a.float() - b.float()).abs().max().item())
+
+
+@pytest.mark.parametrize("name", ["nnedi3-nns16-win8x4", "nnedi3-nns32-win8x4", "nnedi3-nns64-win8x4",
+                                  "nnedi3-nns256-win8x4", "nnedi3-nns32-win8x6", "nnedi3-nns64-win8x6",
+                                  "nnedi3-nns256-win8x6"])
+def test_tall_staged_tiles_equal_short(name):
+    """The one-product FP16 NNEDI3 forms stage 16-row tiles where a launch has enough of them
+    (a 1920x1080 frame, both passes) and 8-row tiles otherwise (a 256x128 crop): the two
+    give bit-identical outputs over the crop's interior."""
+    import mpx
+    x = _frame(1920, 1080, 1, 9)
+    print("tall and short staged tiles", name, flush=True)
+    big = mpx.upscale(name, x, precision="fp16")
+    c0, r0, cw, ch = 800, 400, 256, 128
+    small = mpx.upscale(name, x[r0:r0 + ch, c0:c0 + cw].contiguous(), precision="fp16")
+    a = big[2 * (r0 + M):2 * (r0 + ch - M), 2 * (c0 + M):2 * (c0 + cw - M)]
+    b = small[2 * M:2 * (ch - M), 2 * M:2 * (cw - M)]
+    assert a.shape == b.shape
+    assert torch.equal(a, b), (name, (a.float() - b.float()).abs().max().item())
