@@ -972,10 +972,12 @@ static hipError_t run_pass(const NnediWeights& nw, NPass& p, hipStream_t st, hip
     for (int f = 0; f < p.ntab_src; ++f) p.tiles[f + 1] = p.tiles[f] + ((p.fW[f] + NX - 1) / NX) * ((p.fH[f] + ty - 1) / ty);
     return p.tiles[p.ntab_src];
   };
-  int total = count_tiles(nw.precision == 0 ? NY : NYS);
-  if (nw.precision == 0)
-    return launch_done(nnedi3_valu<WH, AXIS>, dim3(total), dim3(256), st, done, p, static_cast<const float*>(nw.w),
-                       nw.nns, total);
+  if (nw.precision == 0) {
+    const int tiles = count_tiles(NY);
+    return launch_done(nnedi3_valu<WH, AXIS>, dim3(tiles), dim3(256), st, done, p, static_cast<const float*>(nw.w),
+                       nw.nns, tiles);
+  }
+  int total = 0;                         // tiles of the launched MFMA instance (launch_inst)
   hipError_t le = hipErrorInvalidValue;  // the MFMA launch's status
   const int nt = nw.nns / 16;
   const float inv_acc = 1.0f / (nw.xscale * nw.wscale);
