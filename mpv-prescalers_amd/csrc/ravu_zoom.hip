@@ -216,12 +216,12 @@ __global__ __launch_bounds__(256) void zoom_phase_kernel(int4* tab, int in_w, in
 
 // One block per TX x TY output tile, one thread per output pixel (kZoomTiles).  (Kernels
 // without anti-ringing: at least 6 waves per SIMD, a register budget their out-of-line
-// re-decision inherits; r2 anti-ringing kernels: 5, 96 VGPRs with a few bytes of scratch,
-// measured 1-8 % faster than 4 waves at 122-126.)  TAB: the plan has the pre-interpolated
+// re-decision inherits; r2 anti-ringing kernels and r3 ones on 16 x 16 tiles: 5, 96 VGPRs
+// with a few bytes of scratch, measured 1-8 % faster than 4 waves at 118-126.)  TAB: the plan has the pre-interpolated
 // weight table; separate instances keep the per-pixel bilinear fetch's registers out of the
 // table path.
 template <int R, bool AR, int CH, bool TAB, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(AR ? (R == 2 ? 5 : 1) : 6))) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(AR ? ((R == 2 || TX == 16) ? 5 : 1) : 6))) void ravu_zoom_kernel(Io io, const __half* __restrict__ lut,
                                                         const __half* __restrict__ lut_ar,
                                                         const float4* __restrict__ wtab, int nphx, int nphy,
                                                         const int4* __restrict__ phase, int out_w, int out_h) {
