@@ -517,8 +517,10 @@ __global__ __launch_bounds__(256, 2) void nnedi3_mfma(NPass p, const void* __res
   // the one-product FP16 forms on 8x6 windows take the window statistics from shared line
   // statistics (their variance is good to a few ulp; the split forms, which resolve ~22 bits,
   // keep the per-pixel two-pass variance, good to one).  On 8x4 windows the table's build
-  // pass and barrier cost more than the 32-term sums they save (measured 1-3% slower).
-  constexpr bool LINES = M::FUSED_SCALE && TERMS == 1 && WH == 6;
+  // pass and barrier cost more than the 32-term sums they save where a wave holds one or two
+  // neuron tiles (nns 16 / 32: 2-3 % slower) and less from four on (nns 64: -2 %, nns 256:
+  // -1 %).
+  constexpr bool LINES = M::FUSED_SCALE && TERMS == 1 && (WH == 6 || NSL * TPW >= 4);
   __shared__ float tile[2][G::TH * G::TW];
   __shared__ float2 stab[LINES ? LS::N : 1];   // line statistics of the staged tile
   __shared__ __attribute__((aligned(16))) T xhi[NPX * XS];
