@@ -31,7 +31,9 @@ template <int P, int PH, int PW, int NT>
 __device__ __forceinline__ void stage_planes(float* tile, const void* base, int dt, float in_max, float in_rmax,
                                              int64_t pitch, int64_t cstride, int W, int H, int ox, int oy) {
   HaloTile<P, PH, PW, NT> h;
+  __builtin_amdgcn_s_setprio(1);  // the loads go out ahead of other waves' arithmetic
   h.issue(base, dt, pitch, cstride, W, H, ox, oy);
+  __builtin_amdgcn_s_setprio(0);
   h.commit(tile, dt, in_max, in_rmax);
 }
 
@@ -348,8 +350,10 @@ __global__ __launch_bounds__(TX * TY) void ravu_lattice_kernel(Io io, const __ha
   const int64_t plane = static_cast<int64_t>(ft.W) * ft.H;
   {
     HaloTile<P, PH, PW, TX * TY> a, b;  // both tiles' loads in flight together
+    __builtin_amdgcn_s_setprio(1);  // the loads go out ahead of other waves' arithmetic
     a.issue(in, io.in_dt, ft.in_pitch, ft.in_cstride, ft.W, ft.H, x0 - R, y0 - R);
     b.issue(int11 + f * int11_frame, DT_F32, ft.W, plane, ft.W, ft.H, x0 - R, y0 - R);
+    __builtin_amdgcn_s_setprio(0);
     a.commit(hk, io.in_dt, io.in_max, io.in_rmax);
     b.commit(i11, DT_F32, 1.f, 1.f);
   }

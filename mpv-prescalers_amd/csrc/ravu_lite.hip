@@ -37,7 +37,9 @@ __global__ __launch_bounds__(TX * TY) void ravu_lite_kernel(Io io, const __half*
 
   {
     HaloTile<1, PH, PW, NT> h;
+    __builtin_amdgcn_s_setprio(1);  // the loads go out ahead of other waves' arithmetic
     h.issue(in, io.in_dt, ft.in_pitch, 0, ft.W, ft.H, x0 - HALO, y0 - HALO);
+    __builtin_amdgcn_s_setprio(0);
     h.commit(tile, io.in_dt, io.in_max, io.in_rmax);
   }
   if constexpr (AR) {

@@ -123,6 +123,7 @@ __device__ __forceinline__ void stage_used(float* tile, float4* pw, const void* 
   const uintptr_t b = reinterpret_cast<uintptr_t>(base);
   uint32_t v[E], sh[E];
   int li[E];
+  __builtin_amdgcn_s_setprio(1);  // the loads go out ahead of other waves' arithmetic
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = static_cast<int>(threadIdx.x) + NT * e;
@@ -142,6 +143,7 @@ __device__ __forceinline__ void stage_used(float* tile, float4* pw, const void* 
       li[e] = c * PN + r * PW + col;
     }
   }
+  __builtin_amdgcn_s_setprio(0);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     if (li[e] >= 0) {
